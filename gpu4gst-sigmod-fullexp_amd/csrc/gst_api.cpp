@@ -1,0 +1,735 @@
+// C ABI of the gfx950 TrimCDP / D-TrimCDP engine: graph and group upload,
+// HBM-budgeted batching of queries into lockstep slot batches, the
+// graph-captured iteration loop, tree extraction and the backward scatter.
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "../../include/gst.h"
+#include "gst_device.h"
+
+using namespace gst;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(_e == hipErrorOutOfMemory ? GST_ERR_OOM : GST_ERR_DEVICE,            \
+                  std::string(#expr) + ": " + hipGetErrorString(_e));                  \
+  } while (0)
+
+struct DBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t want, bool* grew = nullptr) {
+    if (want <= bytes && p) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    size_t b = want < 256 ? 256 : want;
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) bytes = b;
+    if (grew) *grew = true;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T>
+  T* as() const {
+    return reinterpret_cast<T*>(p);
+  }
+};
+
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+int ilog2(int64_t x) {
+  int s = 0;
+  while ((int64_t(1) << s) < x) ++s;
+  return s;
+}
+
+struct QueryIn {
+  int32_t idx;
+  int32_t k, D;
+  const int32_t* groups;
+};
+
+}  // namespace
+
+struct gst_graph {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int32_t V = 0, Vp = 0;
+  int64_t E = 0;
+  int64_t n_und = 0;
+  int32_t cus = 256;
+  std::vector<int32_t> h_rowp;
+  DBuf rowp, col, wt, eid;
+  bool has_eid = false;
+  // groups
+  int32_t G = 0;
+  std::vector<int64_t> h_goff;
+  DBuf goff, gmem;
+  // options
+  int32_t chunk_shift = 9;
+  int32_t grid = 0;
+  int32_t iters_per_sync = 32;
+  int32_t hop_prune = 1;
+  int32_t profile = 0;
+  int32_t infeasible_cost = INT32_MAX;
+  int32_t max_tree_edges = 4096;
+  int32_t use_graphs = 1;
+  // engine buffers (grow-only)
+  DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr;
+  DBuf out_uv, out_eid, out_n, out_err;
+  uint32_t* h_cnt = nullptr;  // pinned
+  uint32_t wl_cap = 0;
+  int64_t gen = 0;  // bumps whenever a pointer or scalar baked into a captured graph changes
+  std::map<std::pair<int, int>, std::pair<int64_t, hipGraphExec_t>> graphs;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int64_t nch_total() const {
+    int64_t s = 0;
+    const int32_t CH = 1 << chunk_shift;
+    for (int32_t v = 0; v < V; ++v) {
+      int64_t deg = h_rowp[v + 1] - h_rowp[v];
+      int64_t n = (deg + CH - 1) / CH;
+      s += n ? n : 1;
+    }
+    return s;
+  }
+  int64_t nch_cached = -1;
+  int32_t nch_shift = -1;
+  int64_t nch() {
+    if (nch_shift != chunk_shift) {
+      nch_cached = nch_total();
+      nch_shift = chunk_shift;
+    }
+    return nch_cached;
+  }
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.second);
+    graphs.clear();
+  }
+};
+
+namespace {
+
+int32_t rows_of(int32_t k, int32_t D) { return (D < 0 ? 1 : D + 1) << k; }
+
+// Table footprint of one slot (int32 elements) and its worklist share.
+int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
+  const int64_t rows = rows_of(k, D);
+  return rows * g->Vp + std::max<int64_t>(rows, 8) * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
+}
+
+int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
+
+DevCtx make_ctx(gst_graph* g) {
+  DevCtx c;
+  c.rowp = g->rowp.as<int32_t>();
+  c.col = g->col.as<int32_t>();
+  c.wt = g->wt.as<int32_t>();
+  c.eid = g->has_eid ? g->eid.as<int32_t>() : nullptr;
+  c.V = g->V;
+  c.Vp = g->Vp;
+  c.chunk_shift = g->chunk_shift;
+  c.hop_prune = g->hop_prune;
+  c.T = g->T.as<int32_t>();
+  c.E = g->Es.as<int32_t>();
+  c.C = g->C.as<int32_t>();
+  c.R = g->R.as<int32_t>();
+  c.vv = g->VVb.as<int32_t>();
+  c.bestw = g->bestw.as<uint64_t>();
+  c.slots = g->slots.as<SlotDesc>();
+  c.iter = g->iter.as<int32_t>();
+  c.ctr = g->ctr.as<unsigned long long>();
+  c.cap = g->wl_cap;
+  return c;
+}
+
+int expand_grid(const gst_graph* g) { return g->grid > 0 ? g->grid : g->cus * 8; }
+
+// One frontier iteration pair: list 0 -> list 1, list 1 -> list 0.
+void launch_pair(gst_graph* g, const DevCtx& c, int mode) {
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  WL a{g->wl0.as<uint64_t>(), cnt, g->wl1.as<uint64_t>(), cnt + 1};
+  WL b{g->wl1.as<uint64_t>(), cnt + 1, g->wl0.as<uint64_t>(), cnt};
+  const int grid = expand_grid(g);
+  for (int h = 0; h < 2; ++h) {
+    const WL& w = h ? b : a;
+    if (mode == 0) launch_sssp_expand(c, w, grid, g->stream);
+    else launch_dp_expand(c, w, grid, mode == 2, g->stream);
+    launch_advance(c, cnt + h, cnt + (1 - h), mode != 0, g->stream);
+  }
+}
+
+// Runs frontier iterations until the worklist is empty. mode 0 = one-label,
+// 1 = plain DP, 2 = hop DP. Kernel timings are accumulated in profiling mode.
+int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
+  DevCtx c = make_ctx(g);
+  int pairs = 1;
+  const int max_pairs = std::max(1, g->iters_per_sync / 2);
+  int64_t total_pairs = 0;
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  while (true) {
+    if (g->profile) {
+      const int grid = expand_grid(g);
+      WL a{g->wl0.as<uint64_t>(), cnt, g->wl1.as<uint64_t>(), cnt + 1};
+      WL b{g->wl1.as<uint64_t>(), cnt + 1, g->wl0.as<uint64_t>(), cnt};
+      for (int h = 0; h < 2; ++h) {
+        HIP_TRY(hipEventRecord(g->ev0, g->stream));
+        if (mode == 0) launch_sssp_expand(c, h ? b : a, grid, g->stream);
+        else launch_dp_expand(c, h ? b : a, grid, mode == 2, g->stream);
+        HIP_TRY(hipEventRecord(g->ev1, g->stream));
+        launch_advance(c, cnt + h, cnt + (1 - h), mode != 0, g->stream);
+        HIP_TRY(hipEventSynchronize(g->ev1));
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        if (ms_acc) *ms_acc += ms;
+        if (launches) ++*launches;
+      }
+      pairs = 1;
+    } else if (g->use_graphs) {
+      auto key = std::make_pair(mode, pairs);
+      auto itg = g->graphs.find(key);
+      if (itg == g->graphs.end() || itg->second.first != g->gen) {
+        if (itg != g->graphs.end()) {
+          (void)hipGraphExecDestroy(itg->second.second);
+          g->graphs.erase(itg);
+        }
+        hipGraph_t graph;
+        HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+        for (int r = 0; r < pairs; ++r) launch_pair(g, c, mode);
+        HIP_TRY(hipStreamEndCapture(g->stream, &graph));
+        hipGraphExec_t exec;
+        hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(e);
+        g->graphs[key] = std::make_pair(g->gen, exec);
+        itg = g->graphs.find(key);
+      }
+      HIP_TRY(hipGraphLaunch(itg->second.second, g->stream));
+    } else {
+      for (int r = 0; r < pairs; ++r) launch_pair(g, c, mode);
+    }
+    HIP_TRY(hipGetLastError());
+    total_pairs += pairs;
+    HIP_TRY(hipMemcpyAsync(g->h_cnt, cnt, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_cnt + 2, g->ctr.as<uint64_t>() + C_OVERFLOW, sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    uint64_t overflow;
+    std::memcpy(&overflow, g->h_cnt + 2, sizeof(overflow));
+    if (overflow) return 100;  // worklist too small: the batch is re-run with full capacity
+    if (g->h_cnt[0] == 0) break;
+    if (total_pairs > (int64_t)1 << 17)
+      return fail(GST_ERR_INTERNAL, "frontier iteration limit exceeded");
+    pairs = std::min(pairs * 2, max_pairs);
+  }
+  return GST_OK;
+}
+
+// Solves one lockstep batch of same-mode queries. Results go to the caller's
+// per-query vectors. Returns GST_ERR_CAPACITY-like internal retry signal 100
+// when the worklist overflowed.
+struct BatchOut {
+  std::vector<int32_t> cost, status;
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> uv;
+  std::vector<std::vector<int32_t>> ids;
+};
+
+int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac, BatchOut& out,
+                   gst_stats* st) {
+  const int nslots = (int)qs.size();
+  std::vector<SlotDesc> sd(nslots);
+  std::vector<Task> tasks;
+  int64_t t_tot = 0, e_tot = 0, items = 0, sssp_items = 0;
+  for (int s = 0; s < nslots; ++s) {
+    const QueryIn& q = qs[s];
+    SlotDesc d;
+    std::memset(&d, 0, sizeof(d));
+    const int64_t rows = rows_of(q.k, q.D);
+    d.t_off = t_tot;
+    d.e_off = e_tot;
+    d.c_off = (int64_t)s * g->Vp * CROW;
+    d.r_off = (int64_t)s * RTABLE;
+    d.k = q.k;
+    d.full = (1 << q.k) - 1;
+    d.D = q.D;
+    d.nsub = 1 << q.k;
+    d.vv_off = s * VV;
+    t_tot += rows * g->Vp;
+    e_tot += std::max<int64_t>(rows, 8) * g->Vp;
+    items += slot_items(g, q.k, q.D);
+    sssp_items += (int64_t)q.k * g->nch();
+    sd[s] = d;
+    for (int i = 0; i < q.k; ++i) tasks.push_back(Task{s, i, q.groups[i], 0});
+  }
+  int64_t cap = std::max((int64_t)((double)items * wl_frac), sssp_items) + 4096;
+  if (cap > (int64_t)UINT32_MAX - 65536) cap = (int64_t)UINT32_MAX - 65536;
+  bool grew = false;
+  HIP_TRY(g->T.ensure((size_t)t_tot * 4, &grew));
+  bool e_grew = false;
+  HIP_TRY(g->Es.ensure((size_t)e_tot * 4, &e_grew));
+  if (e_grew) HIP_TRY(hipMemsetAsync(g->Es.p, 0, g->Es.bytes, g->stream));
+  grew |= e_grew;
+  HIP_TRY(g->C.ensure((size_t)nslots * g->Vp * CROW * 4, &grew));
+  HIP_TRY(g->R.ensure((size_t)nslots * RTABLE * 4, &grew));
+  HIP_TRY(g->VVb.ensure((size_t)nslots * VV * 4, &grew));
+  HIP_TRY(g->bestw.ensure((size_t)nslots * 8, &grew));
+  HIP_TRY(g->slots.ensure((size_t)nslots * sizeof(SlotDesc), &grew));
+  HIP_TRY(g->tasks.ensure(tasks.size() * sizeof(Task) + 16, &grew));
+  if ((uint64_t)cap * 8 > g->wl0.bytes || (uint64_t)cap * 8 > g->wl1.bytes) {
+    HIP_TRY(g->wl0.ensure((size_t)cap * 8, &grew));
+    HIP_TRY(g->wl1.ensure((size_t)cap * 8, &grew));
+  }
+  const uint32_t new_cap =
+      (uint32_t)std::min<size_t>(std::min(g->wl0.bytes, g->wl1.bytes) / 8, UINT32_MAX - 65536);
+  if (new_cap != g->wl_cap) {
+    g->wl_cap = new_cap;
+    grew = true;
+  }
+  const int maxe = g->max_tree_edges;
+  HIP_TRY(g->out_uv.ensure((size_t)nslots * maxe * 8, &grew));
+  HIP_TRY(g->out_eid.ensure((size_t)nslots * maxe * 4, &grew));
+  HIP_TRY(g->out_n.ensure((size_t)nslots * 4, &grew));
+  HIP_TRY(g->out_err.ensure((size_t)nslots * 4, &grew));
+  if (grew) {
+    ++g->gen;
+    g->drop_graphs();
+  }
+  HIP_TRY(hipMemcpyAsync(g->slots.p, sd.data(), nslots * sizeof(SlotDesc), hipMemcpyHostToDevice,
+                         g->stream));
+  HIP_TRY(hipMemcpyAsync(g->tasks.p, tasks.data(), tasks.size() * sizeof(Task),
+                         hipMemcpyHostToDevice, g->stream));
+  DevCtx c = make_ctx(g);
+  launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
+  launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
+  launch_init_slot_tables(c, nslots, t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
+  const Task* dt = g->tasks.as<Task>();
+  const int nt = (int)tasks.size();
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  launch_groups_setup(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(),
+                      g->wl0.as<uint64_t>(), cnt, g->stream);
+  HIP_TRY(hipGetLastError());
+  double sssp_ms = 0, dp_ms = 0;
+  int64_t dp_launches = 0, sssp_launches = 0;
+  int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
+  if (rc) return rc;  // includes 100 = overflow
+  launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
+  launch_route(c, nslots, g->stream);
+  launch_dp_seed(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(),
+                 cnt, g->stream);
+  HIP_TRY(hipGetLastError());
+  rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
+  if (rc) return rc;
+  launch_backtrack(c, nslots, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
+                   g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), g->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<uint64_t> ctr(C_NUM), bw(nslots);
+  std::vector<int32_t> on(nslots), oerr(nslots);
+  HIP_TRY(hipMemcpyAsync(ctr.data(), g->ctr.p, C_NUM * 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(bw.data(), g->bestw.p, nslots * 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(on.data(), g->out_n.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(oerr.data(), g->out_err.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  if (ctr[C_OVERFLOW]) return 100;  // worklist too small: caller retries with full capacity
+  std::vector<int32_t> huv((size_t)nslots * maxe * 2), hid((size_t)nslots * maxe);
+  HIP_TRY(hipMemcpy(huv.data(), g->out_uv.p, huv.size() * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(hid.data(), g->out_eid.p, hid.size() * 4, hipMemcpyDeviceToHost));
+  for (int s = 0; s < nslots; ++s) {
+    const int qi = qs[s].idx;
+    const int32_t cost = (int32_t)(bw[s] >> 32);
+    if (oerr[s] != 0)
+      return fail(GST_ERR_INTERNAL, "tree recovery failed for query " + std::to_string(qi) +
+                                        " (code " + std::to_string(oerr[s]) + ")");
+    if (cost >= INF) {
+      out.cost[qi] = g->infeasible_cost;
+      out.status[qi] = GST_Q_INFEASIBLE;
+      continue;
+    }
+    out.cost[qi] = cost;
+    out.status[qi] = GST_Q_OK;
+    std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> es;
+    for (int i = 0; i < on[s]; ++i) {
+      int32_t a = huv[((size_t)s * maxe + i) * 2], b = huv[((size_t)s * maxe + i) * 2 + 1];
+      es.push_back({{std::min(a, b), std::max(a, b)}, hid[(size_t)s * maxe + i]});
+    }
+    std::sort(es.begin(), es.end());
+    es.erase(std::unique(es.begin(), es.end(),
+                         [](const auto& x, const auto& y) { return x.first == y.first; }),
+             es.end());
+    out.uv[qi].clear();
+    out.ids[qi].clear();
+    for (auto& e : es) {
+      out.uv[qi].push_back(e.first);
+      out.ids[qi].push_back(e.second);
+    }
+  }
+  if (st) {
+    st->batches += 1;
+    st->queries += nslots;
+    st->sssp_iterations += (int64_t)ctr[C_SSSP_ITERS];
+    st->dp_iterations += (int64_t)ctr[C_DP_ITERS];
+    st->sssp_items += (int64_t)ctr[C_SSSP_ITEMS];
+    st->sssp_relax += (int64_t)ctr[C_SSSP_RELAX];
+    st->states_expanded += (int64_t)ctr[C_F];
+    st->grow_relax += (int64_t)ctr[C_RG];
+    st->merge_relax += (int64_t)ctr[C_RM];
+    st->improvements += (int64_t)ctr[C_U];
+    st->lb_terms += (int64_t)ctr[C_LB];
+    st->items_processed += (int64_t)ctr[C_ITEMS];
+    st->max_frontier = std::max<int64_t>(st->max_frontier, (int64_t)ctr[C_MAX_FRONTIER]);
+    st->expand_ms += dp_ms;
+    st->expand_launches += dp_launches;
+    st->sssp_expand_ms += sssp_ms;
+  }
+  return GST_OK;
+}
+
+int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& out,
+              gst_stats* st) {
+  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  if (budget <= 0) {
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    budget = (int64_t)((double)fr * 0.8);
+  }
+  const double frac = 0.25;
+  // stable order: plain queries first, then hop queries (one mode per batch)
+  std::stable_sort(qs.begin(), qs.end(),
+                   [](const QueryIn& a, const QueryIn& b) { return (a.D >= 0) < (b.D >= 0); });
+  size_t i = 0;
+  while (i < qs.size()) {
+    std::vector<QueryIn> batch;
+    int64_t used = 0;
+    const bool hop = qs[i].D >= 0;
+    while (i < qs.size() && (qs[i].D >= 0) == hop && batch.size() < MAX_SLOTS) {
+      const int64_t f = gst_query_footprint(g, qs[i].k, qs[i].D);
+      if (!batch.empty() && used + f > budget) break;
+      if (batch.empty() && f > budget)
+        return fail(GST_ERR_OOM, "a single query exceeds the HBM budget");
+      used += f;
+      batch.push_back(qs[i]);
+      ++i;
+    }
+    int rc = solve_lockstep(g, batch, frac, out, st);
+    if (rc == 100) rc = solve_lockstep(g, batch, 1.0, out, st);
+    if (rc == 100) return fail(GST_ERR_INTERNAL, "worklist overflow at full capacity");
+    if (rc) return rc;
+  }
+  return GST_OK;
+}
+
+template <class I>
+int upload_graph(gst_graph* g, const I* beg, const I* csr, const I* wgt, int64_t V, int64_t E) {
+  if (V < 1 || V >= (int64_t)MAX_V) return fail(GST_ERR_INVALID, "V out of range (1 .. 2^24-1)");
+  if (E < 0 || E >= (int64_t)INT32_MAX) return fail(GST_ERR_INVALID, "E out of range");
+  if ((int64_t)beg[0] != 0 || (int64_t)beg[V] != E)
+    return fail(GST_ERR_INVALID, "beg_pos must start at 0 and end at E");
+  g->h_rowp.resize(V + 1);
+  std::vector<int32_t> col(E), w(E);
+  uint64_t f_out = 0, f_in = 0;
+  for (int64_t v = 0; v < V; ++v) {
+    if (beg[v + 1] < beg[v]) return fail(GST_ERR_INVALID, "beg_pos not monotone");
+    g->h_rowp[v] = (int32_t)beg[v];
+    for (int64_t j = (int64_t)beg[v]; j < (int64_t)beg[v + 1]; ++j) {
+      const int64_t u = (int64_t)csr[j], ww = (int64_t)wgt[j];
+      if (u < 0 || u >= V) return fail(GST_ERR_INVALID, "neighbour id out of range");
+      if (ww < 1 || ww > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
+      col[j] = (int32_t)u;
+      w[j] = (int32_t)ww;
+      f_out += mix64((uint64_t)v * 0x9E3779B97F4A7C15ull ^ (uint64_t)u * 0xC2B2AE3D27D4EB4Full ^
+                     (uint64_t)ww * 0x165667B19E3779F9ull);
+      f_in += mix64((uint64_t)u * 0x9E3779B97F4A7C15ull ^ (uint64_t)v * 0xC2B2AE3D27D4EB4Full ^
+                    (uint64_t)ww * 0x165667B19E3779F9ull);
+    }
+  }
+  g->h_rowp[V] = (int32_t)E;
+  if (f_out != f_in)
+    return fail(GST_ERR_INVALID, "CSR is not symmetric: every edge must appear in both rows "
+                                 "with the same weight");
+  g->V = (int32_t)V;
+  g->Vp = (int32_t)((V + 63) / 64 * 64);
+  g->E = E;
+  HIP_TRY(g->rowp.ensure((size_t)(V + 1) * 4));
+  HIP_TRY(g->col.ensure((size_t)std::max<int64_t>(E, 1) * 4));
+  HIP_TRY(g->wt.ensure((size_t)std::max<int64_t>(E, 1) * 4));
+  HIP_TRY(hipMemcpy(g->rowp.p, g->h_rowp.data(), (V + 1) * 4, hipMemcpyHostToDevice));
+  if (E) {
+    HIP_TRY(hipMemcpy(g->col.p, col.data(), E * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->wt.p, w.data(), E * 4, hipMemcpyHostToDevice));
+  }
+  for (int64_t v = 0; v < V; ++v) {
+    const int64_t deg = (int64_t)beg[v + 1] - (int64_t)beg[v];
+    if (deg > ((int64_t)MAX_CHUNKS << 6))
+      return fail(GST_ERR_INVALID, "vertex degree above 2^22 is not supported");
+  }
+  return GST_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t gst_version(void) { return 1; }
+
+const char* gst_last_error(void) { return g_last_error.c_str(); }
+
+int gst_graph_create(const void* beg_pos, const void* csr, const void* weight, int64_t V,
+                     int64_t E, int32_t index_bits, gst_graph** out) {
+  if (!out || !beg_pos || (!csr && E) || (!weight && E))
+    return fail(GST_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  HIP_TRY(hipGetDeviceCount(&ndev));
+  if (ndev < 1) return fail(GST_ERR_DEVICE, "no HIP device");
+  gst_graph* g = new gst_graph;
+  HIP_TRY(hipGetDevice(&g->device));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, g->device) == hipSuccess && prop.multiProcessorCount > 0)
+    g->cus = prop.multiProcessorCount;
+  int rc;
+  if (index_bits == 64)
+    rc = upload_graph(g, (const int64_t*)beg_pos, (const int64_t*)csr, (const int64_t*)weight, V, E);
+  else if (index_bits == 32)
+    rc = upload_graph(g, (const int32_t*)beg_pos, (const int32_t*)csr, (const int32_t*)weight, V, E);
+  else
+    rc = fail(GST_ERR_INVALID, "index_bits must be 32 or 64");
+  if (rc) {
+    gst_graph_destroy(g);
+    return rc;
+  }
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
+      hipHostMalloc((void**)&g->h_cnt, 16, 0) != hipSuccess) {
+    gst_graph_destroy(g);
+    return fail(GST_ERR_DEVICE, "stream / event / pinned allocation failed");
+  }
+  if (g->cnt.ensure(16) != hipSuccess || g->iter.ensure(16) != hipSuccess ||
+      g->ctr.ensure(C_NUM * 8) != hipSuccess) {
+    gst_graph_destroy(g);
+    return fail(GST_ERR_OOM, "counter allocation failed");
+  }
+  int32_t one[4] = {1, 0, 0, 0};
+  if (hipMemcpy(g->iter.p, one, 16, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(g->cnt.p, 0, 16) != hipSuccess) {
+    gst_graph_destroy(g);
+    return fail(GST_ERR_DEVICE, "counter initialisation failed");
+  }
+  *out = g;
+  return GST_OK;
+}
+
+int gst_graph_set_edge_ids(gst_graph* g, const int32_t* ids, int64_t num_undirected) {
+  if (!g || !ids || num_undirected < 0) return fail(GST_ERR_INVALID, "bad argument");
+  for (int64_t j = 0; j < g->E; ++j)
+    if (ids[j] < 0 || ids[j] >= num_undirected) return fail(GST_ERR_INVALID, "edge id out of range");
+  HIP_TRY(g->eid.ensure((size_t)std::max<int64_t>(g->E, 1) * 4));
+  if (g->E) HIP_TRY(hipMemcpy(g->eid.p, ids, g->E * 4, hipMemcpyHostToDevice));
+  g->has_eid = true;
+  g->n_und = num_undirected;
+  ++g->gen;
+  return GST_OK;
+}
+
+int gst_graph_set_weights(gst_graph* g, const int32_t* w) {
+  if (!g || !w) return fail(GST_ERR_INVALID, "bad argument");
+  for (int64_t j = 0; j < g->E; ++j)
+    if (w[j] < 1 || w[j] > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
+  if (g->E) HIP_TRY(hipMemcpy(g->wt.p, w, g->E * 4, hipMemcpyHostToDevice));
+  return GST_OK;
+}
+
+int gst_groups_set(gst_graph* g, const int64_t* off, const int32_t* mem, int32_t G) {
+  if (!g || !off || G < 1) return fail(GST_ERR_INVALID, "bad argument");
+  if (off[0] != 0) return fail(GST_ERR_INVALID, "group offsets must start at 0");
+  for (int32_t i = 0; i < G; ++i)
+    if (off[i + 1] < off[i]) return fail(GST_ERR_INVALID, "group offsets not monotone");
+  const int64_t tot = off[G];
+  for (int64_t m = 0; m < tot; ++m)
+    if (mem[m] < 0 || mem[m] >= g->V) return fail(GST_ERR_INVALID, "group member out of range");
+  g->G = G;
+  g->h_goff.assign(off, off + G + 1);
+  HIP_TRY(g->goff.ensure((size_t)(G + 1) * 8));
+  HIP_TRY(g->gmem.ensure((size_t)std::max<int64_t>(tot, 1) * 4));
+  HIP_TRY(hipMemcpy(g->goff.p, off, (G + 1) * 8, hipMemcpyHostToDevice));
+  if (tot) HIP_TRY(hipMemcpy(g->gmem.p, mem, tot * 4, hipMemcpyHostToDevice));
+  ++g->gen;
+  return GST_OK;
+}
+
+int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
+  if (!g) return fail(GST_ERR_INVALID, "null graph");
+  switch (opt) {
+    case GST_OPT_CHUNK:
+      if (v < 64 || v > 65536 || (v & (v - 1))) return fail(GST_ERR_INVALID, "chunk must be a power of two in [64, 65536]");
+      g->chunk_shift = ilog2(v);
+      break;
+    case GST_OPT_GRID:
+      if (v < 0 || v > 65536) return fail(GST_ERR_INVALID, "grid out of range");
+      g->grid = (int32_t)v;
+      break;
+    case GST_OPT_ITERS_PER_SYNC:
+      if (v < 2 || v > 4096) return fail(GST_ERR_INVALID, "iters_per_sync out of range");
+      g->iters_per_sync = (int32_t)v;
+      break;
+    case GST_OPT_HOP_PRUNE:
+      g->hop_prune = v ? 1 : 0;
+      break;
+    case GST_OPT_PROFILE:
+      g->profile = v ? 1 : 0;
+      break;
+    case GST_OPT_INFEASIBLE_COST:
+      if (v < 0 || v > INT32_MAX) return fail(GST_ERR_INVALID, "cost out of range");
+      g->infeasible_cost = (int32_t)v;
+      break;
+    case GST_OPT_MAX_TREE_EDGES:
+      if (v < 1 || v > (1 << 24)) return fail(GST_ERR_INVALID, "max tree edges out of range");
+      g->max_tree_edges = (int32_t)v;
+      break;
+    case GST_OPT_USE_GRAPHS:
+      g->use_graphs = v ? 1 : 0;
+      break;
+    default:
+      return fail(GST_ERR_INVALID, "unknown option");
+  }
+  ++g->gen;
+  g->drop_graphs();
+  return GST_OK;
+}
+
+int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
+  if (!g || k < 1 || k > GST_MAX_K) return -1;
+  const double wl_items = std::max(0.25 * (double)slot_items(g, k, D), (double)k * (double)g->nch());
+  return slot_ints(g, k, D) * 4 + (int64_t)(2 * 8 * wl_items) +
+         (int64_t)g->max_tree_edges * 12 + 1024;
+}
+
+int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                    const int32_t* q_D, int32_t nq, int64_t budget, int32_t* out_costs,
+                    int32_t* out_status, int64_t* out_edge_offsets, int32_t* out_edge_uv,
+                    int32_t* out_edge_ids, int64_t edge_cap, gst_stats* st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!g || nq < 0 || (nq && (!q_offsets || !q_groups || !out_costs)))
+    return fail(GST_ERR_INVALID, "bad argument");
+  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  HIP_TRY(hipSetDevice(g->device));
+  std::vector<QueryIn> qs(nq);
+  for (int32_t q = 0; q < nq; ++q) {
+    const int64_t k = q_offsets[q + 1] - q_offsets[q];
+    const int32_t D = q_D ? q_D[q] : -1;
+    if (k < 1 || k > GST_MAX_K)
+      return fail(GST_ERR_INVALID, "query " + std::to_string(q) + ": k must be in [1, 7]");
+    if (D > GST_MAX_D) return fail(GST_ERR_INVALID, "hop bound above 15");
+    const int32_t* gr = q_groups + q_offsets[q];
+    for (int64_t i = 0; i < k; ++i) {
+      if (gr[i] < 0 || gr[i] >= g->G)
+        return fail(GST_ERR_INVALID, "query " + std::to_string(q) + ": group id out of range");
+      for (int64_t j = 0; j < i; ++j)
+        if (gr[j] == gr[i])
+          return fail(GST_ERR_INVALID, "query " + std::to_string(q) + ": repeated group id");
+    }
+    qs[q] = QueryIn{q, (int32_t)k, D < 0 ? -1 : D, gr};
+  }
+  BatchOut out;
+  out.cost.assign(nq, 0);
+  out.status.assign(nq, 0);
+  out.uv.resize(nq);
+  out.ids.resize(nq);
+  gst_stats local;
+  std::memset(&local, 0, sizeof(local));
+  int rc = nq ? solve_all(g, qs, budget, out, &local) : GST_OK;
+  if (rc) return rc;
+  int64_t pos = 0;
+  bool over = false;
+  for (int32_t q = 0; q < nq; ++q) {
+    out_costs[q] = out.cost[q];
+    if (out_status) out_status[q] = out.status[q];
+    if (out_edge_offsets) out_edge_offsets[q] = pos;
+    for (size_t i = 0; i < out.uv[q].size(); ++i, ++pos) {
+      if (pos >= edge_cap) {
+        over = true;
+        continue;
+      }
+      if (out_edge_uv) {
+        out_edge_uv[2 * pos] = out.uv[q][i].first;
+        out_edge_uv[2 * pos + 1] = out.uv[q][i].second;
+      }
+      if (out_edge_ids) out_edge_ids[pos] = out.ids[q][i];
+    }
+  }
+  if (out_edge_offsets) out_edge_offsets[nq] = pos;
+  local.seconds_total =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  local.bytes_alg = 16 * local.states_expanded + 12 * local.grow_relax + 8 * local.merge_relax +
+                    12 * local.improvements + 4 * local.lb_terms;
+  if (st) *st = local;
+  if (over && (out_edge_uv || out_edge_ids))
+    return fail(GST_ERR_CAPACITY, "edge_cap too small; offsets hold the needed size");
+  return GST_OK;
+}
+
+int gst_solve(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D, int32_t* out_cost,
+              int32_t* out_edge_uv, int32_t edge_cap, int32_t* out_num_edges, int32_t* out_status,
+              gst_stats* st) {
+  if (!g || !query_groups || !out_cost) return fail(GST_ERR_INVALID, "bad argument");
+  int64_t off[2] = {0, k};
+  int64_t eoff[2] = {0, 0};
+  int32_t status = 0;
+  int rc = gst_solve_batch(g, off, query_groups, &D, 1, 0, out_cost, &status, eoff, out_edge_uv,
+                           nullptr, out_edge_uv ? edge_cap : 0, st);
+  if (out_num_edges) *out_num_edges = (int32_t)eoff[1];
+  if (out_status) *out_status = status;
+  return rc;
+}
+
+int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* d_ids,
+                 const float* d_gout, float* d_gw, int64_t nund) {
+  if (!g || nq < 0 || nund < 0) return fail(GST_ERR_INVALID, "bad argument");
+  if (nq == 0) return GST_OK;
+  if (!d_off || !d_ids || !d_gout || !d_gw) return fail(GST_ERR_INVALID, "null pointer");
+  HIP_TRY(hipSetDevice(g->device));
+  launch_backward(nq, d_off, d_ids, d_gout, d_gw, nund, g->stream);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return GST_OK;
+}
+
+void gst_graph_destroy(gst_graph* g) {
+  if (!g) return;
+  g->drop_graphs();
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es,
+                  &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
+                  &g->cnt, &g->iter, &g->ctr, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
+  for (DBuf* b : bufs) b->release();
+  if (g->h_cnt) (void)hipHostFree(g->h_cnt);
+  if (g->ev0) (void)hipEventDestroy(g->ev0);
+  if (g->ev1) (void)hipEventDestroy(g->ev1);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+}  // extern "C"

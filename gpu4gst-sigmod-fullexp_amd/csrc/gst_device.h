@@ -1,0 +1,132 @@
+// Shared host/device definitions of the gfx950 TrimCDP / D-TrimCDP engine.
+#pragma once
+
+#ifdef GST_CPU_EMU  // CPU emulation build for fault finding (emu/hip_cpu_emu.h)
+#include "emu/hip_cpu_emu.h"
+#else
+#include <hip/hip_runtime.h>
+#endif
+#include <stdint.h>
+
+namespace gst {
+
+// "Unreachable / no tree yet". Sums of three such values stay below 2^31,
+// so bound checks of the form a + b + c never overflow (the reference's
+// inf = 100000, code/TrimCDP/include/DPQ.cuh:50, is not safe for that).
+constexpr int32_t INF = 1 << 29;
+constexpr int BLOCK = 256;          // threads per workgroup of the expand kernels
+constexpr int WAVE = 64;            // gfx950 wavefront
+constexpr int RSTRIDE = 72;         // ints per subset in the route tables: 8x8 route + 8 min-route
+constexpr int RTABLE = 128 * RSTRIDE;  // route ints per query slot (2^7 subsets)
+constexpr int CROW = 8;             // one-label distances per vertex (k <= 7, padded to 8)
+constexpr int VV = 64;              // 8x8 group-to-group distances per slot
+
+// Worklist item: one frontier state (or one chunk of a high-degree state).
+//   bits  0..23  vertex v        bits 24..31  group set p (one-label phase: group index)
+//   bits 32..35  hop layer d     bits 36..51  chunk index of v's CSR row
+//   bits 52..63  query slot
+__host__ __device__ inline uint64_t mk_item(uint32_t v, uint32_t p, uint32_t d, uint32_t ch,
+                                            uint32_t s) {
+  return (uint64_t)v | ((uint64_t)p << 24) | ((uint64_t)d << 32) | ((uint64_t)ch << 36) |
+         ((uint64_t)s << 52);
+}
+__host__ __device__ inline uint32_t it_v(uint64_t x) { return (uint32_t)(x & 0xFFFFFFu); }
+__host__ __device__ inline uint32_t it_p(uint64_t x) { return (uint32_t)((x >> 24) & 0xFFu); }
+__host__ __device__ inline uint32_t it_d(uint64_t x) { return (uint32_t)((x >> 32) & 0xFu); }
+__host__ __device__ inline uint32_t it_ch(uint64_t x) { return (uint32_t)((x >> 36) & 0xFFFFu); }
+__host__ __device__ inline uint32_t it_s(uint64_t x) { return (uint32_t)(x >> 52); }
+constexpr uint32_t MAX_V = 1u << 24;
+constexpr uint32_t MAX_SLOTS = 1u << 12;
+constexpr uint32_t MAX_CHUNKS = 1u << 16;
+
+// Best-solution word of a query: (cost << 32) | witness, updated with a
+// 64-bit atomicMin so the cost and a state that reproduces it move together.
+//   witness bit 31: 0 = the full state T(v, full[, d]) itself,
+//                   1 = T(v, p) plus shortest paths to the groups outside p
+//   bits 24..30: p (one-label witness) or d (full witness), bits 0..23: v
+__host__ __device__ inline uint32_t wit_full(uint32_t v, uint32_t d) { return (d << 24) | v; }
+__host__ __device__ inline uint32_t wit_onelabel(uint32_t v, uint32_t p) {
+  return 0x80000000u | (p << 24) | v;
+}
+
+// Per-query-slot layout; all offsets count int32 elements of their arena.
+struct SlotDesc {
+  int64_t t_off;   // cost table T[(d * nsub + p) * Vp + v]; row 0 holds the group mask of v
+  int64_t e_off;   // enqueue stamps, same indexing as T (rows 0..k-1 serve the one-label phase)
+  int64_t c_off;   // one-label distances c[v * 8 + i]
+  int64_t r_off;   // route tables [p][72]
+  int32_t k, full, D, nsub;  // D < 0 = plain problem
+  int32_t vv_off;  // group distance matrix [8][8] in the vv array
+  int32_t pad[3];
+};
+
+// Counters accumulated on device (uint64 each).
+enum Ctr {
+  C_SSSP_ITEMS = 0,
+  C_SSSP_RELAX,
+  C_F,
+  C_RG,
+  C_RM,
+  C_U,
+  C_LB,
+  C_ITEMS,
+  C_MAX_FRONTIER,
+  C_DP_ITERS,
+  C_SSSP_ITERS,
+  C_OVERFLOW,
+  C_NUM
+};
+
+struct DevCtx {
+  const int32_t* rowp;  // [V+1]
+  const int32_t* col;   // [E]
+  const int32_t* wt;    // [E]
+  const int32_t* eid;   // [E] undirected edge id (may be null = CSR index)
+  int32_t V, Vp;
+  int32_t chunk_shift;  // virtual-split chunk = 1 << chunk_shift edges
+  int32_t hop_prune;
+  int32_t* T;
+  int32_t* E;
+  int32_t* C;
+  int32_t* R;
+  int32_t* vv;
+  uint64_t* bestw;
+  const SlotDesc* slots;
+  int32_t* iter;                 // current iteration number (stamps use iter + 1)
+  unsigned long long* ctr;       // Ctr counters
+  uint32_t cap;                  // worklist capacity (items)
+};
+
+struct WL {
+  const uint64_t* in;
+  const uint32_t* in_cnt;
+  uint64_t* out;
+  uint32_t* out_cnt;
+};
+
+// Host-side launchers (gst_kernels.hip).
+struct Task {  // one (slot, query group) pair for member-driven kernels
+  int32_t slot, bit, group, pad;
+};
+
+void launch_fill_i32(int32_t* p, int64_t n, int32_t val, hipStream_t st);
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t val, hipStream_t st);
+void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+                         const int32_t* gmem, uint64_t* wl, uint32_t* cnt, hipStream_t st);
+void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st);
+void launch_dp_expand(const DevCtx& c, const WL& w, int grid, bool hop, hipStream_t st);
+void launch_advance(const DevCtx& c, uint32_t* cnt_consumed, const uint32_t* cnt_next, bool dp,
+                    hipStream_t st);
+void launch_vv(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+               const int32_t* gmem, hipStream_t st);
+void launch_route(const DevCtx& c, int nslots, hipStream_t st);
+void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+                    const int32_t* gmem, uint64_t* wl, uint32_t* cnt, hipStream_t st);
+void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_uv,
+                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, hipStream_t st);
+void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
+                     float* gw, int64_t nund, hipStream_t st);
+void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
+                             hipStream_t st);
+
+}  // namespace gst

@@ -1,0 +1,931 @@
+// gfx950 kernels of the TrimCDP / D-TrimCDP engine.
+//
+// Phases of a batch of query slots (all slots advance in lockstep):
+//   1. k_groups_setup   group mask of every member vertex, one-label seeds
+//   2. k_sssp_expand    label-correcting multi-source shortest paths c_i(v)
+//                       (reference: dis0_init_1, code/TrimCDP/src/DPQ.cu:260-290,
+//                       and hybrid_bin_scan_push_kernel_only,
+//                       code/TrimCDP-WB/include/mapper_enactor.cuh:249-366)
+//   3. k_vv, k_route    group-to-group distances and route tables for the tour
+//                       lower bounds (host code at code/TrimCDP/src/DPQ.cu:426-501)
+//   4. k_dp_seed        T(v, p) = 0 for p inside v's groups (DPQ.cu:349-389)
+//   5. k_dp_expand      the DP frontier step: upper bound, 1/2 rule, merge over
+//                       submask pairs, grow over CSR rows with int32 atomicMin,
+//                       lower-bound filtered enqueue (Relax, DPQ.cu:86-191;
+//                       hop variant: code/D-TrimCDP/src/DPQ.cu:62-124)
+//   6. k_backtrack      tree recovery from the final table
+//   7. k_backward       edge-incidence subgradient scatter
+//
+// Frontier bookkeeping is the same in every expand kernel: a state enters the
+// next worklist only if atomicMax of its stamp to (iteration + 1) raised it,
+// so each state is queued at most once per iteration without any clearing
+// pass, and appends are aggregated per wavefront (ballot / wave scan, one
+// atomicAdd per wave). Rows longer than one chunk are queued as one item per
+// chunk (virtual split), and each workgroup balances the edges of its 256
+// items with an LDS prefix scan, so high- and low-degree rows share the same
+// coalesced edge loop.
+#include "gst_device.h"
+
+namespace gst {
+
+namespace {
+
+__device__ inline int lane_id() { return __lane_id(); }
+
+// Appends `cnt` chunk items (item | chunk << 36) per lane; every lane of the
+// wavefront must call this together.
+__device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, uint32_t* out_cnt,
+                                   uint32_t cap, unsigned long long* ctr) {
+  const int lane = lane_id();
+  const uint64_t many = __ballot(cnt > 1);
+  uint32_t excl, total;
+  if (many == 0) {
+    const uint64_t m = __ballot(cnt == 1);
+    if (m == 0) return;
+    total = (uint32_t)__popcll(m);
+    excl = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  } else {
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+      uint32_t y = __shfl_up(incl, off, WAVE);
+      if (lane >= off) incl += y;
+    }
+    total = __shfl(incl, WAVE - 1, WAVE);
+    excl = incl - cnt;
+  }
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(out_cnt, total);
+  base = __shfl(base, 0, WAVE);
+  for (uint32_t i = 0; i < cnt; ++i) {
+    uint32_t pos = base + excl + i;
+    if (pos < cap) out[pos] = item | ((uint64_t)i << 36);
+    else atomicAdd(&ctr[C_OVERFLOW], 1ull);
+  }
+}
+
+__device__ inline void ub_update(uint64_t* bw, int32_t cost, uint32_t wit) {
+  const uint64_t pk = ((uint64_t)(uint32_t)cost << 32) | wit;
+  if (pk < __hip_atomic_load(bw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+    atomicMin((unsigned long long*)bw, (unsigned long long)pk);
+}
+
+__device__ inline int32_t best_of(const uint64_t* bw) {
+  return (int32_t)(__hip_atomic_load(bw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >> 32);
+}
+
+__device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
+  const int32_t deg = c.rowp[u + 1] - c.rowp[u];
+  const uint32_t n = (uint32_t)((deg + (1 << c.chunk_shift) - 1) >> c.chunk_shift);
+  return n ? n : 1u;
+}
+
+// Lower bound of any tree that contains u and touches every group in U
+// (U != 0), from the one-label distances cs = c_i(u): the largest distance,
+// and the two tour bounds of PrunedDP++ (code/PrunedDP++/include/
+// CPUNONHOP.h:433-493) over exact group-route tables R1[i][j] = cheapest
+// group sequence i ... j visiting all of U, R2[i] = min_j R1[i][j].
+// Also returns sum_{i in U} c_i(u) (capped at INF) for the one-label upper
+// bound. Returns INF when some group of U is unreachable from u.
+__device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_t U,
+                                  int32_t* sumc) {
+  const int4 a = *reinterpret_cast<const int4*>(crow);
+  const int4 b = *reinterpret_cast<const int4*>(crow + 4);
+  const int32_t cs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  int32_t one = 0, mn = INF;
+  int64_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if ((U >> i) & 1u) {
+      one = max(one, cs[i]);
+      mn = min(mn, cs[i]);
+      sum += cs[i];
+    }
+  if (one >= INF) {
+    *sumc = INF;
+    return INF;
+  }
+  *sumc = sum >= INF ? INF : (int32_t)sum;
+  int32_t lb1 = INF, lb2 = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    if (!((U >> i) & 1u)) continue;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if ((U >> j) & 1u) lb1 = min(lb1, (cs[i] + R[i * 8 + j] + cs[j]) >> 1);
+    lb2 = max(lb2, (cs[i] + R[64 + i] + mn) >> 1);
+  }
+  return max(one, max(lb1, lb2));
+}
+
+// Exclusive prefix sum over the 256 threads of a workgroup; s_off[256] = total.
+__device__ inline void block_scan(int32_t x, int32_t* s_off, int32_t* s_wsum) {
+  const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
+  int32_t incl = x;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    int32_t y = __shfl_up(incl, off, WAVE);
+    if (lane >= off) incl += y;
+  }
+  if (lane == WAVE - 1) s_wsum[wid] = incl;
+  __syncthreads();
+  int32_t prefix = 0;
+#pragma unroll
+  for (int w = 0; w < BLOCK / WAVE; ++w)
+    if (w < wid) prefix += s_wsum[w];
+  s_off[tid] = prefix + incl - x;
+  if (tid == BLOCK - 1) s_off[BLOCK] = prefix + incl;
+  __syncthreads();
+}
+
+__device__ inline int find_item(const int32_t* s_off, int32_t e) {
+  int lo = 0, hi = BLOCK;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (s_off[mid] <= e) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline void flush(unsigned long long* ctr, int idx, unsigned long long v) {
+  // wave-reduce, then one atomic per wave
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_down(v, off, WAVE);
+  if (lane_id() == 0 && v) atomicAdd(&ctr[idx], v);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// fills and per-batch initialisation
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+
+__global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t val) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = val;
+}
+
+// Row 0 of every slot's table holds the query-group mask of each vertex.
+__global__ void k_zero_row0(DevCtx c) {
+  const SlotDesc S = c.slots[blockIdx.y];
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.Vp; v += gridDim.x * blockDim.x)
+    c.T[S.t_off + v] = 0;
+}
+
+void launch_fill_i32(int32_t* p, int64_t n, int32_t val, hipStream_t st) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_i32, (int)blocks, 256, 0, st, p, n, val);
+}
+
+void launch_fill_u64(uint64_t* p, int64_t n, uint64_t val, hipStream_t st) {
+  if (n <= 0) return;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u64, (int)blocks, 256, 0, st, p, n, val);
+}
+
+void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
+                             hipStream_t st) {
+  launch_fill_i32(c.T, t_total, INF, st);
+  launch_fill_i32(c.C, c_total, INF, st);
+  launch_fill_i32(c.vv, (int64_t)nslots * VV, INF, st);
+  launch_fill_u64(c.bestw, nslots, ((uint64_t)INF << 32) | 0xFFFFFFFFull, st);
+  dim3 grid((c.Vp + 255) / 256 < 1024 ? (c.Vp + 255) / 256 : 1024, nslots);
+  hipLaunchKernelGGL(k_zero_row0, grid, 256, 0, st, c);
+}
+
+// ---------------------------------------------------------------------------
+// group masks and one-label seeds: c_i(v) = 0 for members of query group i
+
+__global__ __launch_bounds__(BLOCK) void k_groups_setup(DevCtx c, const Task* tasks, int ntasks,
+                                                        const int64_t* goff, const int32_t* gmem,
+                                                        uint64_t* wl, uint32_t* cnt) {
+  const int32_t stamp = *c.iter;
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const Task tk = tasks[t];
+    const SlotDesc S = c.slots[tk.slot];
+    const int64_t b = goff[tk.group], e = goff[tk.group + 1];
+    for (int64_t base = b; base < e; base += BLOCK) {
+      const int64_t m = base + threadIdx.x;
+      uint32_t want = 0;
+      uint32_t v = 0;
+      if (m < e) {
+        v = (uint32_t)gmem[m];
+        atomicOr(&c.T[S.t_off + v], 1 << tk.bit);
+        atomicMin(&c.C[S.c_off + (int64_t)v * CROW + tk.bit], 0);
+        int32_t* st = &c.E[S.e_off + (int64_t)tk.bit * c.Vp + v];
+        if (atomicMax(st, stamp) < stamp) want = nchunks(c, v);
+      }
+      wave_append(want, mk_item(v, (uint32_t)tk.bit, 0, 0, (uint32_t)tk.slot), wl, cnt, c.cap,
+                  c.ctr);
+    }
+  }
+}
+
+void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+                         const int32_t* gmem, uint64_t* wl, uint32_t* cnt, hipStream_t st) {
+  if (ntasks <= 0) return;
+  int grid = ntasks < 4096 ? ntasks : 4096;
+  hipLaunchKernelGGL(k_groups_setup, grid, BLOCK, 0, st, c, tasks, ntasks, goff, gmem, wl, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// one-label phase: label-correcting multi-source shortest paths, all groups of
+// all slots in one worklist. Item (v, i): relax c_i over v's CSR chunk.
+
+__global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
+  __shared__ int32_t s_off[BLOCK + 1];
+  __shared__ int32_t s_wsum[BLOCK / WAVE];
+  __shared__ int32_t s_beg[BLOCK];
+  __shared__ int32_t s_cost[BLOCK];
+  __shared__ int64_t s_cbase[BLOCK];   // C offset of column i for the item's slot
+  __shared__ int64_t s_ebase[BLOCK];   // stamp row base
+  __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | i
+  const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
+  const int32_t next = *c.iter + 1;
+  const int tid = threadIdx.x;
+  const int32_t CH = 1 << c.chunk_shift;
+  unsigned long long n_items = 0, n_relax = 0;
+  for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
+    const uint32_t idx = tile * BLOCK + tid;
+    int32_t deg = 0;
+    if (idx < n) {
+      const uint64_t it = w.in[idx];
+      const uint32_t v = it_v(it), i = it_p(it), ch = it_ch(it), s = it_s(it);
+      const SlotDesc S = c.slots[s];
+      const int64_t cb = S.c_off + i;
+      const int32_t t = c.C[cb + (int64_t)v * CROW];
+      const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
+      const int32_t re = min(c.rowp[v + 1], rb + CH);
+      deg = t < INF ? max(0, re - rb) : 0;
+      s_beg[tid] = rb;
+      s_cost[tid] = t;
+      s_cbase[tid] = cb;
+      s_ebase[tid] = S.e_off + (int64_t)i * c.Vp;
+      s_meta[tid] = (s << 8) | i;
+      ++n_items;
+    }
+    block_scan(deg, s_off, s_wsum);
+    const int32_t total = s_off[BLOCK];
+    for (int32_t base = 0; base < total; base += BLOCK) {
+      const int32_t e = base + tid;
+      uint32_t cnt = 0;
+      uint64_t item = 0;
+      if (e < total) {
+        const int j = find_item(s_off, e);
+        const int32_t ej = s_beg[j] + (e - s_off[j]);
+        const uint32_t u = (uint32_t)c.col[ej];
+        const int32_t cost = s_cost[j] + c.wt[ej];
+        int32_t* dst = &c.C[s_cbase[j] + (int64_t)u * CROW];
+        ++n_relax;
+        if (cost < *dst && cost < atomicMin(dst, cost)) {
+          int32_t* st = &c.E[s_ebase[j] + u];
+          if (atomicMax(st, next) < next) {
+            cnt = nchunks(c, u);
+            item = mk_item(u, s_meta[j] & 0xFFu, 0, 0, s_meta[j] >> 8);
+          }
+        }
+      }
+      wave_append(cnt, item, w.out, w.out_cnt, c.cap, c.ctr);
+    }
+    __syncthreads();
+  }
+  flush(c.ctr, C_SSSP_ITEMS, n_items);
+  flush(c.ctr, C_SSSP_RELAX, n_relax);
+}
+
+void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_sssp_expand, grid, BLOCK, 0, st, c, w);
+}
+
+// Consumes one iteration: clears the consumed list's counter, advances the
+// iteration number and records frontier statistics.
+__global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_next, int dp) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    const uint32_t produced = *cnt_next;
+    if (*cnt_consumed > 0) atomicAdd(&c.ctr[dp ? C_DP_ITERS : C_SSSP_ITERS], 1ull);
+    if (dp && produced > c.ctr[C_MAX_FRONTIER]) c.ctr[C_MAX_FRONTIER] = produced;
+    *cnt_consumed = 0;
+    *c.iter += 1;
+  }
+}
+
+void launch_advance(const DevCtx& c, uint32_t* cnt_consumed, const uint32_t* cnt_next, bool dp,
+                    hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, 1, 64, 0, st, c, cnt_consumed, cnt_next, dp ? 1 : 0);
+}
+
+// ---------------------------------------------------------------------------
+// tour-bound tables
+
+// vv[i][j] = min over members v of query group i of c_j(v).
+__global__ __launch_bounds__(BLOCK) void k_vv(DevCtx c, const Task* tasks, int ntasks,
+                                              const int64_t* goff, const int32_t* gmem) {
+  __shared__ int32_t s_min[8];
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const Task tk = tasks[t];
+    const SlotDesc S = c.slots[tk.slot];
+    if (threadIdx.x < 8) s_min[threadIdx.x] = INF;
+    __syncthreads();
+    int32_t loc[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) loc[j] = INF;
+    for (int64_t m = goff[tk.group] + threadIdx.x; m < goff[tk.group + 1]; m += BLOCK) {
+      const int32_t* row = &c.C[S.c_off + (int64_t)gmem[m] * CROW];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) loc[j] = min(loc[j], row[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      int32_t x = loc[j];
+#pragma unroll
+      for (int off = WAVE / 2; off > 0; off >>= 1) x = min(x, __shfl_down(x, off, WAVE));
+      if (lane_id() == 0) atomicMin(&s_min[j], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8 && (int)threadIdx.x < S.k)
+      atomicMin(&c.vv[S.vv_off + tk.bit * 8 + threadIdx.x], s_min[threadIdx.x]);
+    __syncthreads();
+  }
+}
+
+void launch_vv(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+               const int32_t* gmem, hipStream_t st) {
+  if (ntasks <= 0) return;
+  int grid = ntasks < 4096 ? ntasks : 4096;
+  hipLaunchKernelGGL(k_vv, grid, BLOCK, 0, st, c, tasks, ntasks, goff, gmem);
+}
+
+// Held-Karp over the query groups of one slot: rt[i][j][X] = cheapest
+// sequence of groups starting at i, ending at j, visiting exactly X (i, j in X,
+// i != j unless X = {i}), with vv as the step cost. Then R1[X][i][j] = rt and
+// R2[X][i] = min_j rt[i][j][X].
+__global__ __launch_bounds__(64) void k_route(DevCtx c) {
+  __shared__ int32_t rt[8 * 8 * 128];
+  __shared__ int32_t vv[64];
+  const SlotDesc S = c.slots[blockIdx.x];
+  const int k = S.k, nsub = S.nsub;
+  const int tid = threadIdx.x;
+  for (int x = tid; x < 8 * 8 * 128; x += 64) rt[x] = INF;
+  vv[tid] = c.vv[S.vv_off + tid];
+  __syncthreads();
+  if (tid < k) rt[(tid * 8 + tid) * 128 + (1 << tid)] = 0;
+  __syncthreads();
+  const int i = tid >> 3, j = tid & 7;
+  for (int X = 1; X < nsub; ++X) {
+    if (i < k && j < k && i != j && ((X >> i) & 1) && ((X >> j) & 1)) {
+      const int Y = X ^ (1 << j);
+      int32_t best = INF;
+      for (int m = 0; m < k; ++m)
+        if ((Y >> m) & 1) best = min(best, rt[(i * 8 + m) * 128 + Y] + vv[m * 8 + j]);
+      rt[(i * 8 + j) * 128 + X] = min(best, INF);
+    }
+    __syncthreads();
+  }
+  int32_t* R = c.R + S.r_off;
+  for (int X = 0; X < nsub; ++X) {
+    R[X * RSTRIDE + tid] = (i < k && j < k) ? rt[(i * 8 + j) * 128 + X] : INF;
+    if (tid < 8) {
+      int32_t m = INF;
+      for (int jj = 0; jj < k; ++jj) m = min(m, rt[(tid * 8 + jj) * 128 + X]);
+      R[X * RSTRIDE + 64 + tid] = m;
+    }
+  }
+}
+
+void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
+  if (nslots > 0) hipLaunchKernelGGL(k_route, nslots, 64, 0, st, c);
+}
+
+// ---------------------------------------------------------------------------
+// DP seeds: T(v, p[, 0]) = 0 for every non-empty p inside v's query groups.
+// Each vertex is seeded from its lowest query group only.
+
+__global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, int ntasks,
+                                                   const int64_t* goff, const int32_t* gmem,
+                                                   uint64_t* wl, uint32_t* cnt) {
+  const int32_t stamp = *c.iter;
+  for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
+    const Task tk = tasks[t];
+    const SlotDesc S = c.slots[tk.slot];
+    const bool hop = S.D >= 0;
+    const int64_t b = goff[tk.group], e = goff[tk.group + 1];
+    for (int64_t base = b; base < e; base += BLOCK) {
+      const int64_t m = base + threadIdx.x;
+      uint32_t v = 0, gsv = 0;
+      if (m < e) {
+        v = (uint32_t)gmem[m];
+        gsv = (uint32_t)c.T[S.t_off + v];
+        if ((uint32_t)__builtin_ctz(gsv | 0x100u) != (uint32_t)tk.bit) gsv = 0;
+      }
+      uint32_t p = gsv;  // enumerate non-empty submasks of gsv
+      bool live = p != 0;
+      while (__any(live)) {
+        uint32_t want = 0;
+        uint64_t item = 0;
+        if (live) {
+          atomicMin(&c.T[S.t_off + (int64_t)p * c.Vp + v], 0);
+          if ((int32_t)p == S.full) {
+            ub_update(&c.bestw[tk.slot], 0, wit_full(v, 0));
+          } else {
+            if (!hop) {
+              int32_t sumc;
+              const uint32_t U = (uint32_t)S.full ^ p;
+              (void)lb_eval(&c.C[S.c_off + (int64_t)v * CROW], &c.R[S.r_off + U * RSTRIDE], U,
+                            &sumc);
+              if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
+            }
+            if (atomicMax(&c.E[S.e_off + (int64_t)p * c.Vp + v], stamp) < stamp) {
+              want = nchunks(c, v);
+              item = mk_item(v, p, 0, 0, (uint32_t)tk.slot);
+            }
+          }
+          p = (p - 1) & gsv;
+          live = p != 0;
+        }
+        wave_append(want, item, wl, cnt, c.cap, c.ctr);
+      }
+    }
+  }
+}
+
+void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
+                    const int32_t* gmem, uint64_t* wl, uint32_t* cnt, hipStream_t st) {
+  if (ntasks <= 0) return;
+  int grid = ntasks < 4096 ? ntasks : 4096;
+  hipLaunchKernelGGL(k_dp_seed, grid, BLOCK, 0, st, c, tasks, ntasks, goff, gmem, wl, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// The DP frontier step.
+//
+// Plain (TrimCDP) item (v, p) with t = T(v, p), best = the query's upper bound:
+//   * complement: T(v, full) <- t + T(v, full ^ p); best <- that   (DPQ.cu:95-108)
+//   * 1/2 rule: expand only if t < best/2 + 2                       (DPQ.cu:127;
+//     the +2 safeguard of code/TrimCDP-multi-core-CPU/include/CPUNONHOP.h:741)
+//   * merge with every finite T(v, p2), p2 a proper submask of full ^ p; an
+//     improved T(v, P) is queued if 1000c <= 667 best + 2000 and c + lb <= best + 2
+//   * grow over the CSR chunk: T(u, p) <- t + w(v, u); queued if c + lb <= best + 2
+//   * every improvement also offers c + sum_{i not in P} c_i(u) as an upper bound
+// Hop (D-TrimCDP) item (v, p, d):
+//   * merge with T(v, p2, d2), p2 a non-empty submask of full ^ p, d2 <= D - d,
+//     into (v, p | p2, max(d, d2)); a full result updates best
+//   * grow (d < D) into T(u, p, d + 1)
+//   * with hop pruning on, an improvement is queued only if c + lb <= best
+//     (lb from hop-free distances is a valid bound for hop-bounded trees)
+
+template <bool HOP>
+__global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
+  __shared__ int32_t s_off[BLOCK + 1];
+  __shared__ int32_t s_wsum[BLOCK / WAVE];
+  __shared__ int32_t s_beg[BLOCK];
+  __shared__ int32_t s_cost[BLOCK];
+  __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row
+  __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
+  const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
+  const int32_t next = *c.iter + 1;
+  const int tid = threadIdx.x;
+  const int32_t CH = 1 << c.chunk_shift;
+  const int64_t Vp = c.Vp;
+  unsigned long long nF = 0, nRg = 0, nRm = 0, nU = 0, nLb = 0, nItems = 0;
+
+  for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
+    const uint32_t idx = tile * BLOCK + tid;
+    int32_t deg = 0;
+    uint32_t v = 0, p = 0, d = 0, s = 0, mask = 0;
+    int32_t t = INF, Dq = 0;
+    SlotDesc S;
+    S.t_off = S.e_off = S.c_off = S.r_off = 0;
+    S.k = S.full = S.D = S.nsub = S.vv_off = 0;
+    if (idx < n) {
+      ++nItems;
+      const uint64_t it = w.in[idx];
+      v = it_v(it);
+      p = it_p(it);
+      d = it_d(it);
+      s = it_s(it);
+      const uint32_t ch = it_ch(it);
+      S = c.slots[s];
+      Dq = S.D;
+      const int64_t row = HOP ? (int64_t)d * S.nsub + p : (int64_t)p;
+      t = c.T[S.t_off + row * Vp + v];
+      bool expand = t < INF;
+      if (!HOP) {
+        if (ch == 0) {
+          const uint32_t xs = (uint32_t)S.full ^ p;
+          const int32_t t2 = c.T[S.t_off + (int64_t)xs * Vp + v];
+          if (t2 < INF && t < INF) {
+            const int32_t mcost = t + t2;
+            int32_t* dfull = &c.T[S.t_off + (int64_t)S.full * Vp + v];
+            if (mcost < *dfull && mcost < atomicMin(dfull, mcost))
+              ub_update(&c.bestw[s], mcost, wit_full(v, 0));
+          }
+        }
+        const int32_t best = best_of(&c.bestw[s]);
+        expand = expand && t < best / 2 + 2;
+      }
+      if (expand) {
+        nF += ch == 0;
+        const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
+        const int32_t re = min(c.rowp[v + 1], rb + CH);
+        deg = (HOP && (int32_t)d >= Dq) ? 0 : max(0, re - rb);
+        s_beg[tid] = rb;
+        s_cost[tid] = t;
+        s_dst[tid] = S.t_off + (HOP ? ((int64_t)(d + 1) * S.nsub + p) : (int64_t)p) * Vp;
+        s_meta[tid] = (s << 12) | (d << 8) | p;
+        if (ch == 0) mask = (uint32_t)S.full ^ p;
+      }
+    }
+
+    // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
+    {
+      uint32_t p2, d2 = 0;
+      if (HOP) p2 = mask;                 // every non-empty submask, complement included
+      else p2 = (mask - 1u) & mask;       // proper submasks; the complement was done above
+      bool live = p2 != 0;
+      while (__any(live)) {
+        uint32_t want = 0;
+        uint64_t item = 0;
+        if (live) {
+          ++nRm;
+          const int64_t src = HOP ? ((int64_t)d2 * S.nsub + p2) : (int64_t)p2;
+          const int32_t t2 = c.T[S.t_off + src * Vp + v];
+          if (t2 < INF) {
+            const int32_t mc = t + t2;
+            const uint32_t P = p | p2;
+            const uint32_t nd = HOP ? max(d, d2) : 0u;
+            const int64_t drow = HOP ? ((int64_t)nd * S.nsub + P) : (int64_t)P;
+            int32_t* dst = &c.T[S.t_off + drow * Vp + v];
+            if (mc < *dst && mc < atomicMin(dst, mc)) {
+              ++nU;
+              if ((int32_t)P == S.full) {
+                ub_update(&c.bestw[s], mc, wit_full(v, nd));
+              } else {
+                const uint32_t U = (uint32_t)S.full ^ P;
+                bool ok = true;
+                if (!HOP || c.hop_prune) {
+                  int32_t sumc;
+                  const int32_t lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW],
+                                             &c.R[S.r_off + U * RSTRIDE], U, &sumc);
+                  nLb += __popc(U);
+                  int32_t best = best_of(&c.bestw[s]);
+                  if (!HOP && sumc < INF && mc + sumc < best) {
+                    ub_update(&c.bestw[s], mc + sumc, wit_onelabel(v, P));
+                    best = mc + sumc;
+                  }
+                  if (HOP) ok = lb < INF && mc + lb <= best;
+                  else
+                    ok = lb < INF && (int64_t)1000 * mc <= (int64_t)667 * best + 2000 &&
+                         mc + lb <= best + 2;
+                }
+                if (ok && atomicMax(&c.E[S.e_off + drow * Vp + v], next) < next) {
+                  want = nchunks(c, v);
+                  item = mk_item(v, P, nd, 0, s);
+                }
+              }
+            }
+          }
+          if (HOP) {
+            if ((int32_t)++d2 > Dq - (int32_t)d) {
+              d2 = 0;
+              p2 = (p2 - 1u) & mask;
+            }
+          } else {
+            p2 = (p2 - 1u) & mask;
+          }
+          live = p2 != 0;
+        }
+        wave_append(want, item, w.out, w.out_cnt, c.cap, c.ctr);
+      }
+    }
+
+    // ---- grow: the workgroup's edges, balanced by an LDS prefix scan
+    block_scan(deg, s_off, s_wsum);
+    const int32_t total = s_off[BLOCK];
+    for (int32_t base = 0; base < total; base += BLOCK) {
+      const int32_t e = base + tid;
+      uint32_t want = 0;
+      uint64_t item = 0;
+      if (e < total) {
+        const int j = find_item(s_off, e);
+        const int32_t ej = s_beg[j] + (e - s_off[j]);
+        const uint32_t u = (uint32_t)c.col[ej];
+        const int32_t gc = s_cost[j] + c.wt[ej];
+        const int64_t dbase = s_dst[j];
+        int32_t* dst = &c.T[dbase + u];
+        ++nRg;
+        if (gc < *dst && gc < atomicMin(dst, gc)) {
+          ++nU;
+          const uint32_t meta = s_meta[j];
+          const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
+          const SlotDesc Sj = c.slots[sl];
+          const uint32_t U = (uint32_t)Sj.full ^ pd;
+          bool ok = true;
+          if (!HOP || c.hop_prune) {
+            int32_t sumc;
+            const int32_t lb = lb_eval(&c.C[Sj.c_off + (int64_t)u * CROW],
+                                       &c.R[Sj.r_off + U * RSTRIDE], U, &sumc);
+            nLb += __popc(U);
+            int32_t best = best_of(&c.bestw[sl]);
+            if (!HOP && sumc < INF && gc + sumc < best) {
+              ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+              best = gc + sumc;
+            }
+            ok = HOP ? (lb < INF && gc + lb <= best) : (lb < INF && gc + lb <= best + 2);
+          }
+          const int64_t erow = dbase - Sj.t_off + Sj.e_off;
+          if (ok && atomicMax(&c.E[erow + u], next) < next) {
+            want = nchunks(c, u);
+            item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+          }
+        }
+      }
+      wave_append(want, item, w.out, w.out_cnt, c.cap, c.ctr);
+    }
+    __syncthreads();
+  }
+  flush(c.ctr, C_F, nF);
+  flush(c.ctr, C_RG, nRg);
+  flush(c.ctr, C_RM, nRm);
+  flush(c.ctr, C_U, nU);
+  flush(c.ctr, C_LB, nLb);
+  flush(c.ctr, C_ITEMS, nItems);
+}
+
+void launch_dp_expand(const DevCtx& c, const WL& w, int grid, bool hop, hipStream_t st) {
+  if (hop) hipLaunchKernelGGL(k_dp_expand<true>, grid, BLOCK, 0, st, c, w);
+  else hipLaunchKernelGGL(k_dp_expand<false>, grid, BLOCK, 0, st, c, w);
+}
+
+// ---------------------------------------------------------------------------
+// Tree recovery. One wavefront per slot walks back from the best word's
+// witness. A state (v, p[, d]) with budget b (its current table value) is a
+// base state when p lies inside v's groups; otherwise some merge split with
+// T(v,p1[,d1]) + T(v,p2[,d2]) <= b or some neighbour u with
+// T(u,p[,d-1]) + w(u,v) <= b exists, because every table value was written
+// from predecessor values that can only have decreased since. Recursing on
+// such predecessors yields an edge set of weight <= cost; at the optimum it
+// is an optimal tree. Among candidates the lowest index is taken.
+
+namespace {
+constexpr int STACK = 2048;
+
+__device__ inline uint32_t deposit(uint32_t r, uint32_t mask) {  // r-th submask of mask
+  uint32_t out = 0;
+  for (uint32_t b = 0; mask; ++b) {
+    const uint32_t low = mask & (0u - mask);
+    if ((r >> b) & 1u) out |= low;
+    mask ^= low;
+  }
+  return out;
+}
+}  // namespace
+
+__global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32_t* out_uv,
+                                                  int32_t* out_eid, int32_t* out_n,
+                                                  int32_t* out_err) {
+  __shared__ int32_t st_v[STACK], st_p[STACK], st_d[STACK], st_b[STACK];
+  __shared__ int32_t s_sp, s_ne, s_err;
+  __shared__ int32_t s_pick;
+  const int s = blockIdx.x;
+  const int lane = threadIdx.x;
+  const SlotDesc S = c.slots[s];
+  const int64_t Vp = c.Vp;
+  const uint64_t bw = c.bestw[s];
+  const int32_t cost = (int32_t)(bw >> 32);
+  const uint32_t wit = (uint32_t)bw;
+  int32_t* ouv = out_uv + (int64_t)s * max_edges * 2;
+  int32_t* oid = out_eid + (int64_t)s * max_edges;
+  const bool hop = S.D >= 0;
+  if (lane == 0) {
+    s_sp = 0;
+    s_ne = 0;
+    s_err = 0;
+  }
+  __syncthreads();
+  if (cost >= INF) {
+    if (lane == 0) {
+      out_n[s] = 0;
+      out_err[s] = 0;
+    }
+    return;
+  }
+  auto Tat = [&](uint32_t v, uint32_t p, uint32_t d) -> int32_t {
+    return c.T[S.t_off + ((int64_t)d * S.nsub + p) * Vp + v];
+  };
+  auto emit = [&](uint32_t a, uint32_t b2, int32_t ej) {  // lane 0 only
+    if (s_ne >= max_edges) {
+      s_err = 2;
+      return;
+    }
+    ouv[2 * s_ne] = (int32_t)a;
+    ouv[2 * s_ne + 1] = (int32_t)b2;
+    oid[s_ne] = c.eid ? c.eid[ej] : ej;
+    ++s_ne;
+  };
+  const uint32_t rv = wit & 0xFFFFFFu, rx = (wit >> 24) & 0x7Fu;
+  const bool onelabel = (wit >> 31) != 0;
+  if (lane == 0) {
+    if (onelabel) {
+      st_v[0] = (int32_t)rv;
+      st_p[0] = (int32_t)rx;
+      st_d[0] = 0;
+      st_b[0] = Tat(rv, rx, 0);
+    } else {
+      const uint32_t d = hop ? rx : 0u;
+      st_v[0] = (int32_t)rv;
+      st_p[0] = S.full;
+      st_d[0] = (int32_t)d;
+      st_b[0] = Tat(rv, (uint32_t)S.full, d);
+    }
+    s_sp = 1;
+  }
+  __syncthreads();
+  // shortest paths from the witness vertex to the groups outside p
+  if (onelabel) {
+    const uint32_t U = (uint32_t)S.full ^ rx;
+    for (int i = 0; i < S.k; ++i) {
+      if (!((U >> i) & 1u)) continue;
+      uint32_t x = rv;
+      for (int guard = 0; guard < c.V && s_err == 0; ++guard) {
+        const int32_t cx = c.C[S.c_off + (int64_t)x * CROW + i];
+        if (cx == 0) break;
+        const int32_t rb = c.rowp[x], re = c.rowp[x + 1];
+        int32_t pick = -1;
+        for (int32_t jb = rb; jb < re && pick < 0; jb += 64) {
+          const int32_t j = jb + lane;
+          bool ok = false;
+          if (j < re) {
+            const uint32_t y = (uint32_t)c.col[j];
+            ok = c.C[S.c_off + (int64_t)y * CROW + i] + c.wt[j] <= cx;
+          }
+          const uint64_t m = __ballot(ok);
+          if (m) pick = jb + (int32_t)__ffsll((unsigned long long)m) - 1;
+        }
+        if (pick < 0) {
+          if (lane == 0) s_err = 4;
+          __syncthreads();
+          break;
+        }
+        const uint32_t y = (uint32_t)c.col[pick];
+        if (lane == 0) emit(x, y, pick);
+        __syncthreads();
+        x = y;
+      }
+    }
+  }
+  for (int guard = 0;; ++guard) {
+    __syncthreads();
+    if (s_sp == 0 || s_err) break;
+    if (guard > 8 * max_edges + 4096) {  // cannot happen for a consistent table
+      if (lane == 0) s_err = 5;
+      __syncthreads();
+      break;
+    }
+    const int top = s_sp - 1;
+    const uint32_t v = (uint32_t)st_v[top], p = (uint32_t)st_p[top], d = (uint32_t)st_d[top];
+    const int32_t b = st_b[top];
+    __syncthreads();
+    if (lane == 0) s_sp = top;
+    const uint32_t gs = (uint32_t)c.T[S.t_off + v];
+    __syncthreads();
+    if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
+    // merge splits: proper non-empty submasks p1 (hop: with layers d1, d2,
+    // max(d1, d2) = d, d1 + d2 <= D)
+    const uint32_t pc = (uint32_t)__popc(p);
+    const uint32_t nsplit = (1u << pc) - 2u;
+    const uint32_t npair = hop ? 2u * d + 1u : 1u;
+    const uint32_t ncand = nsplit * npair;
+    int32_t pick = -1;
+    for (uint32_t cb = 0; cb < ncand && pick < 0; cb += 64) {
+      const uint32_t r = cb + lane;
+      bool ok = false;
+      if (r < ncand) {
+        const uint32_t p1 = deposit(r / npair + 1u, p), p2 = p ^ p1;
+        uint32_t d1 = 0, d2 = 0;
+        bool legal = true;
+        if (hop) {
+          const uint32_t q = r % npair;  // (d, 0..d) then (0..d-1, d)
+          if (q <= d) {
+            d1 = d;
+            d2 = q;
+          } else {
+            d1 = q - d - 1;
+            d2 = d;
+          }
+          legal = (int32_t)(d1 + d2) <= S.D;
+        }
+        if (legal) {
+          const int32_t a1 = Tat(v, p1, d1), a2 = Tat(v, p2, d2);
+          ok = a1 < INF && a2 < INF && a1 + a2 <= b;
+        }
+      }
+      const uint64_t m = __ballot(ok);
+      if (m) pick = (int32_t)(cb + __ffsll((unsigned long long)m) - 1);
+    }
+    if (pick >= 0) {
+      if (lane == 0) {
+        const uint32_t r = (uint32_t)pick;
+        const uint32_t p1 = deposit(r / npair + 1u, p), p2 = p ^ p1;
+        uint32_t d1 = 0, d2 = 0;
+        if (hop) {
+          const uint32_t q = r % npair;
+          if (q <= d) {
+            d1 = d;
+            d2 = q;
+          } else {
+            d1 = q - d - 1;
+            d2 = d;
+          }
+        }
+        if (s_sp + 2 > STACK) {
+          s_err = 3;
+        } else {
+          st_v[s_sp] = (int32_t)v;
+          st_p[s_sp] = (int32_t)p1;
+          st_d[s_sp] = (int32_t)d1;
+          st_b[s_sp] = Tat(v, p1, d1);
+          st_v[s_sp + 1] = (int32_t)v;
+          st_p[s_sp + 1] = (int32_t)p2;
+          st_d[s_sp + 1] = (int32_t)d2;
+          st_b[s_sp + 1] = Tat(v, p2, d2);
+          s_sp += 2;
+        }
+      }
+      continue;
+    }
+    // grow: a neighbour u with T(u, p[, d-1]) + w <= b
+    if (!hop || d >= 1) {
+      const uint32_t pd = hop ? d - 1 : 0;
+      const int32_t rb = c.rowp[v], re = c.rowp[v + 1];
+      for (int32_t jb = rb; jb < re && pick < 0; jb += 64) {
+        const int32_t j = jb + lane;
+        bool ok = false;
+        if (j < re) {
+          const int32_t tu = Tat((uint32_t)c.col[j], p, pd);
+          ok = tu < INF && tu + c.wt[j] <= b;
+        }
+        const uint64_t m = __ballot(ok);
+        if (m) pick = jb + (int32_t)__ffsll((unsigned long long)m) - 1;
+      }
+      if (pick >= 0) {
+        if (lane == 0) {
+          const uint32_t u = (uint32_t)c.col[pick];
+          emit(v, u, pick);
+          if (s_sp + 1 > STACK) {
+            s_err = 3;
+          } else {
+            st_v[s_sp] = (int32_t)u;
+            st_p[s_sp] = (int32_t)p;
+            st_d[s_sp] = (int32_t)pd;
+            st_b[s_sp] = Tat(u, p, pd);
+            s_sp += 1;
+          }
+        }
+        continue;
+      }
+    }
+    if (lane == 0) s_err = 1;  // no predecessor: table inconsistent
+  }
+  __syncthreads();
+  if (lane == 0) {
+    out_n[s] = s_ne;
+    out_err[s] = s_err;
+  }
+}
+
+void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_uv,
+                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, hipStream_t st) {
+  if (nslots > 0) hipLaunchKernelGGL(k_backtrack, nslots, 64, 0, st, c, max_edges, out_uv, out_eid, out_n, out_err);
+}
+
+// ---------------------------------------------------------------------------
+// backward: grad_w[e] += grad_out[q] for every tree edge e of query q
+
+__global__ void k_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
+                           float* gw, int64_t nund) {
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    const float g = gout[q];
+    for (int64_t i = off[q] + threadIdx.x; i < off[q + 1]; i += blockDim.x) {
+      const int32_t e = ids[i];
+      if (e >= 0 && e < nund) atomicAdd(&gw[e], g);
+    }
+  }
+}
+
+void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
+                     float* gw, int64_t nund, hipStream_t st) {
+  if (nq <= 0) return;
+  int grid = nq < 4096 ? nq : 4096;
+  hipLaunchKernelGGL(k_backward, grid, 256, 0, st, nq, off, ids, gout, gw, nund);
+}
+
+}  // namespace gst

@@ -1,0 +1,154 @@
+/* gst.h - MI355X (gfx950) group Steiner tree solver: TrimCDP and D-TrimCDP.
+ *
+ * C ABI of libgst.so. Plain pointers and sizes only; every entry point
+ * returns a status code and never exits the process. Host arrays passed in
+ * are read during the call only and stay owned by the caller.
+ *
+ * The reference has no library boundary: each solver is a C++ free function
+ * linked into its own executable. The entry points below replace these
+ * reference interfaces (paths relative to the reference root):
+ *
+ *   gst_graph_create   the CSR the reference solvers read:
+ *                      CSR_graph / toCSR (code/TrimCDP/include/graph_v_of_v_idealID/
+ *                      csr_graph.hpp:14-31, 39-128) and the int64 *_beg_pos/_csr/_weight.bin
+ *                      arrays of the work-balanced solvers (code/TrimCDP-WB/include/
+ *                      graph.hpp:7-118, code/TrimCDP-WB/src/GSTnonHop.cu:167-188)
+ *   gst_groups_set     group_graph, built by read_Group (code/TrimCDP/include/
+ *                      graph_hash_of_mixed_weighted_read_for_GSTP.h:190-239)
+ *   gst_solve          DP_GPU (code/TrimCDP/include/DPQ.cuh:67, src/DPQ.cu:292) when D < 0;
+ *                      DP_gpu (code/D-TrimCDP/include/DPQ.cuh:66, src/DPQ.cu:147) when D >= 0;
+ *                      also graph_v_of_v_idealID_PrunedDPPlusPlus
+ *                      (code/PrunedDP++/include/CPUNONHOP.h:496) and HOP_cpu
+ *                      (code/D-PrunedDP++/include/CPUHOP.h:107), whose tree output it adds
+ *   gst_solve_batch    the per-query harness loops exp_GPU_nonHop / exp_GPU_Hop
+ *                      (code/TrimCDP/include/exp_GPU_nonHop.h:40-144,
+ *                      code/D-TrimCDP/include/exp_GPU_Hop.h:40-126) and the query loop of
+ *                      the work-balanced main() (code/TrimCDP-WB/src/GSTnonHop.cu:247-497)
+ *   gst_backward       no reference counterpart (the reference is not differentiable);
+ *                      backs torch.autograd.Function of the Python wrapper
+ *   gst_graph_destroy  the cudaFree sequences (code/TrimCDP/src/DPQ.cu:644-654)
+ *
+ * Semantics kept from the reference: bit i of a group set is query_groups[i]
+ * (code/TrimCDP/src/DPQ.cu:40-52); a vertex is the root of T(v,p) = 0 for every
+ * p contained in the query groups it belongs to; the optimum is
+ * min over v of T(v, full) (plain) or min over v, d <= D of T(v, full, d) (hop).
+ * Costs are exact int32. Weights must be positive.
+ */
+#ifndef GST_H
+#define GST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define GST_OK 0
+#define GST_ERR_INVALID 1    /* bad argument (sizes, ids, weights <= 0, k out of range) */
+#define GST_ERR_OOM 2        /* device allocation failed or a query exceeds the HBM budget */
+#define GST_ERR_DEVICE 3     /* HIP runtime error */
+#define GST_ERR_CAPACITY 4   /* caller's output buffer too small; sizes were written */
+#define GST_ERR_INTERNAL 5   /* consistency check failed inside the solver */
+
+/* per-query status in gst_solve_batch's out_status */
+#define GST_Q_OK 0
+#define GST_Q_INFEASIBLE 1   /* no tree covers all query groups */
+
+#define GST_MAX_K 7           /* up to 128 group subsets per query */
+#define GST_MAX_D 15          /* hop bound */
+
+typedef struct gst_graph gst_graph;
+
+typedef struct gst_stats {
+  int64_t queries;
+  int64_t batches;            /* admitted batches (HBM budget rounds) */
+  int64_t sssp_iterations;    /* frontier iterations of the one-label phase */
+  int64_t dp_iterations;      /* frontier iterations of the DP phase */
+  int64_t sssp_items;         /* one-label states expanded */
+  int64_t sssp_relax;         /* one-label edge relaxations */
+  int64_t states_expanded;    /* F: DP states whose grow/merge ran */
+  int64_t grow_relax;         /* Rg: DP edge relaxations */
+  int64_t merge_relax;        /* Rm: DP merge candidates evaluated */
+  int64_t improvements;       /* U: successful DP table decreases */
+  int64_t lb_terms;           /* Lb: one-label reads made by lower-bound checks */
+  int64_t items_processed;    /* DP worklist items (chunks included) */
+  int64_t max_frontier;       /* largest DP worklist */
+  int64_t bytes_alg;          /* algorithmic bytes of the DP expand kernel */
+  double seconds_total;       /* host wall clock of the call */
+  double expand_ms;           /* DP expand kernel time (profiling mode only) */
+  int64_t expand_launches;    /* DP expand launches (profiling mode only) */
+  double sssp_expand_ms;      /* one-label expand kernel time (profiling mode only) */
+} gst_stats;
+
+/* options for gst_set_option */
+#define GST_OPT_CHUNK 1            /* edges per virtual-split chunk (power of two, 64..65536) */
+#define GST_OPT_GRID 2             /* workgroups of the expand kernels (0 = auto) */
+#define GST_OPT_ITERS_PER_SYNC 3   /* max graph-captured iterations between host checks */
+#define GST_OPT_HOP_PRUNE 4        /* 1 = lower-bound pruning in the hop DP (default 1) */
+#define GST_OPT_PROFILE 5          /* 1 = time every expand launch with HIP events */
+#define GST_OPT_INFEASIBLE_COST 6  /* cost written for infeasible queries (default INT32_MAX;
+                                      100000 reproduces the reference's inf) */
+#define GST_OPT_MAX_TREE_EDGES 7   /* per-query edge capacity of the backtracking kernel */
+#define GST_OPT_USE_GRAPHS 8       /* 1 = capture iteration steps in HIP graphs (default 1) */
+#define GST_OPT_BLOCK_ITEMS 9      /* reserved */
+
+/* Creates a graph on the current HIP device from a symmetric CSR:
+ * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge
+ * appears in both rows. index_bits = 64 reads int64 arrays (the reference's
+ * .bin layout), 32 reads int32 arrays. Weights must be >= 1. */
+int gst_graph_create(const void* beg_pos, const void* csr, const void* weight, int64_t V,
+                     int64_t E, int32_t index_bits, gst_graph** out);
+/* Maps each CSR entry to an undirected edge id in [0, num_undirected); used
+ * for tree edge ids and by gst_backward. Without it the id is the CSR index. */
+int gst_graph_set_edge_ids(gst_graph* g, const int32_t* csr_edge_id, int64_t num_undirected);
+/* Replaces the weights (CSR order, int32, >= 1). */
+int gst_graph_set_weights(gst_graph* g, const int32_t* weight);
+/* Vertex groups: members of group i are group_members[group_offsets[i] ..
+ * group_offsets[i+1]); group ids are 0-based (".g" line "gN" is id N-1). */
+int gst_groups_set(gst_graph* g, const int64_t* group_offsets, const int32_t* group_members,
+                   int32_t G);
+int gst_set_option(gst_graph* g, int32_t option, int64_t value);
+
+/* One query: k (1..GST_MAX_K) group ids; D < 0 = plain TrimCDP, D >= 0 =
+ * hop-bounded D-TrimCDP. Writes the optimal cost, the tree as vertex pairs
+ * (out_edge_uv[2*i], out_edge_uv[2*i+1]) up to edge_cap edges, and the edge
+ * count. Infeasible queries return GST_OK with *out_status = GST_Q_INFEASIBLE
+ * and no edges. out_edge_uv / out_stats / out_status may be NULL. */
+int gst_solve(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D, int32_t* out_cost,
+              int32_t* out_edge_uv, int32_t edge_cap, int32_t* out_num_edges,
+              int32_t* out_status, gst_stats* out_stats);
+
+/* nq queries in one call: query q is q_groups[q_offsets[q] .. q_offsets[q+1]),
+ * with hop bound q_D[q] (q_D may be NULL = all plain). Queries are admitted in
+ * batches whose device footprint fits hbm_budget_bytes (0 = 80% of free HBM)
+ * and solved concurrently. Trees go to out_edge_uv (pairs) / out_edge_ids
+ * (undirected ids, may be NULL), query q at [out_edge_offsets[q],
+ * out_edge_offsets[q+1]). If edge_cap is too small GST_ERR_CAPACITY is
+ * returned after costs and offsets are written. */
+int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                    const int32_t* q_D, int32_t nq, int64_t hbm_budget_bytes, int32_t* out_costs,
+                    int32_t* out_status, int64_t* out_edge_offsets, int32_t* out_edge_uv,
+                    int32_t* out_edge_ids, int64_t edge_cap, gst_stats* out_stats);
+
+/* Backward of cost_q with respect to the undirected edge weights: device
+ * pointers only. grad_w[e] += sum over queries q of grad_out[q] for every tree
+ * edge e of query q (tree q = edge_ids[edge_offsets[q] .. edge_offsets[q+1])).
+ * Runs on the graph's stream and waits for it. */
+int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_edge_offsets,
+                 const int32_t* d_edge_ids, const float* d_grad_out, float* d_grad_w,
+                 int64_t num_undirected);
+
+/* Number of device bytes one query of k groups and hop bound D occupies. */
+int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D);
+/* Thread-local description of the last error. */
+const char* gst_last_error(void);
+void gst_graph_destroy(gst_graph* g);
+/* Library version, for the loader check. */
+int32_t gst_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GST_H */

@@ -1,0 +1,960 @@
+// CPU oracle for optimal group Steiner trees (plain and hop-bounded).
+//
+// TEST INFRASTRUCTURE ONLY. This file is a plain C++ restatement of the
+// reference's CPU solvers, used as the checker by tests/, by
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg. The product
+// library never links, loads or calls it.
+//
+// What is restated, and from where (paths relative to the reference root):
+//   orc_sssp       multi-source Dijkstra from one group (the dummy-source trick of
+//                  code/PrunedDP++/include/CPUNONHOP.h:21-67 and
+//                  graph_v_of_v_idealID_shortest_paths.h:24-81)
+//   orc_prunedpp   PrunedDP++ best-first search (CPUNONHOP.h:496-964), its
+//                  virtual distances (172-207), AllPaths W (266-383), W2
+//                  (565-584), LB procedure (433-493), feasible-solution UB with
+//                  MST post-processing (681-723), build_tree (387-429).
+//                  Boost's Fibonacci heap is replaced by a binary heap with
+//                  explicit stale-entry tracking (same decrease-key semantics)
+//                  and the string-keyed hash maps by arrays ("array mode").
+//   orc_hop        D-PrunedDP++ HOP_cpu (code/D-PrunedDP++/include/CPUHOP.h:107-283).
+//   orc_trimcdp_mt TrimCDP-multi-core-CPU cpu_mt
+//                  (code/TrimCDP-multi-core-CPU/include/CPUNONHOP.h:573-842), a
+//                  level-synchronous label-correcting DP over a thread pool.
+//   orc_dpbf       textbook DPBF (Ding et al., ICDE'07): Dijkstra over (v, p)
+//                  without pruning; exact.
+//   orc_brute_*    exhaustive search on tiny graphs (vertex subsets + MST for the
+//                  plain problem, edge subsets forming trees of bounded hop
+//                  diameter for the D problem).
+//   orc_verify_tree feasibility check of a returned tree
+//                  (code/PrunedDP++/include/exp_CPU_nonHop.h:17-49) plus
+//                  acyclicity, weight sum and hop diameter.
+//
+// Parity status of each function (how it is pinned) is recorded in the test
+// suite: orc_dpbf and orc_hop against brute force and closed forms on random
+// tiny graphs; orc_prunedpp and orc_trimcdp_mt against orc_dpbf.
+//
+// Conventions: CSR int64 beg[V+1], int32 col/w (symmetric, positive weights);
+// groups CSR int64 goff[G+1], int32 gmem; a query is k group ids, bit i of a
+// group-set mask is query group i. ORC_INF = 2^29 marks "unreachable".
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdint>
+#include <cstring>
+#include <functional>
+#include <queue>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+namespace {
+
+constexpr int32_t INF = 1 << 29;
+constexpr int64_t INF64 = (int64_t)1 << 40;
+
+enum Status { OK = 0, INFEASIBLE = 1, INVALID = 2, CAPACITY = 3 };
+
+struct Graph {
+  int32_t V;
+  const int64_t* beg;
+  const int32_t* col;
+  const int32_t* w;
+  int32_t edge_weight(int32_t a, int32_t b) const {
+    int32_t best = -1;
+    for (int64_t j = beg[a]; j < beg[a + 1]; ++j)
+      if (col[j] == b && (best < 0 || w[j] < best)) best = w[j];
+    return best;
+  }
+};
+
+struct Query {
+  int32_t k, full;
+  std::vector<int32_t> gs;  // group-set mask per vertex
+};
+
+bool make_query(const Graph& g, const int64_t* goff, const int32_t* gmem, int32_t G,
+                int32_t k, const int32_t* q, Query& out) {
+  if (k < 1 || k > 20) return false;
+  out.k = k;
+  out.full = (1 << k) - 1;
+  out.gs.assign(g.V, 0);
+  for (int32_t i = 0; i < k; ++i) {
+    if (q[i] < 0 || q[i] >= G) return false;
+    for (int64_t m = goff[q[i]]; m < goff[q[i] + 1]; ++m) {
+      int32_t v = gmem[m];
+      if (v < 0 || v >= g.V) return false;
+      out.gs[v] |= 1 << i;
+    }
+  }
+  return true;
+}
+
+// Multi-source Dijkstra from every vertex in `sources`; pred[v] == v for a
+// source or an unreached vertex.
+void dijkstra(const Graph& g, const std::vector<int32_t>& sources,
+              std::vector<int32_t>& dist, std::vector<int32_t>& pred) {
+  dist.assign(g.V, INF);
+  pred.resize(g.V);
+  for (int32_t v = 0; v < g.V; ++v) pred[v] = v;
+  typedef std::pair<int64_t, int32_t> E;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  for (int32_t s : sources) {
+    if (dist[s] != 0) {
+      dist[s] = 0;
+      pq.push({0, s});
+    }
+  }
+  while (!pq.empty()) {
+    auto [d, v] = pq.top();
+    pq.pop();
+    if (d != dist[v]) continue;
+    for (int64_t j = g.beg[v]; j < g.beg[v + 1]; ++j) {
+      int32_t u = g.col[j];
+      int64_t nd = d + g.w[j];
+      if (nd < dist[u] && nd < INF) {
+        dist[u] = (int32_t)nd;
+        pred[u] = v;
+        pq.push({nd, u});
+      }
+    }
+  }
+}
+
+// One-label distances c_i(v) for the k query groups.
+void one_label(const Graph& g, const int64_t* goff, const int32_t* gmem, int32_t k,
+               const int32_t* q, std::vector<std::vector<int32_t>>& dist,
+               std::vector<std::vector<int32_t>>& pred) {
+  dist.resize(k);
+  pred.resize(k);
+  for (int32_t i = 0; i < k; ++i) {
+    std::vector<int32_t> src(gmem + goff[q[i]], gmem + goff[q[i] + 1]);
+    dijkstra(g, src, dist[i], pred[i]);
+  }
+}
+
+int32_t sat(int64_t x) { return x >= INF ? INF : (int32_t)x; }
+
+// Tour tables of PrunedDP++: virtual distances between groups, AllPaths W
+// (Dijkstra over (start group, current group, visited set) with the subset
+// closure of CPUNONHOP.h:303-318) and W2[i][X] = min_j W[i][j][X].
+struct Tours {
+  int32_t k, width;
+  std::vector<int64_t> W;   // [i][j][X]
+  std::vector<int64_t> W2;  // [i][X]
+  int64_t w(int32_t i, int32_t j, int32_t X) const { return W[((int64_t)i * k + j) * width + X]; }
+};
+
+void build_tours(const Graph& g, const int64_t* goff, const int32_t* gmem, int32_t k,
+                 const int32_t* q, const std::vector<std::vector<int32_t>>& dist, Tours& t) {
+  (void)g;
+  t.k = k;
+  t.width = 1 << k;
+  std::vector<int64_t> vd(k * k);
+  for (int32_t a = 0; a < k; ++a)
+    for (int32_t b = 0; b < k; ++b) {
+      int64_t best = INF;
+      for (int64_t m = goff[q[b]]; m < goff[q[b] + 1]; ++m)
+        best = std::min<int64_t>(best, dist[a][gmem[m]]);
+      vd[a * k + b] = best;
+    }
+  const int64_t nstate = (int64_t)k * k * t.width;
+  t.W.assign(nstate, INF64);
+  std::vector<int64_t> inq(nstate, -1);
+  std::vector<uint8_t> done(nstate, 0);
+  typedef std::tuple<int64_t, int32_t, int32_t, int32_t> E;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  auto id = [&](int32_t i, int32_t j, int32_t X) { return ((int64_t)i * k + j) * t.width + X; };
+  for (int32_t i = 0; i < k; ++i) {
+    pq.push({0, i, i, 0});
+    inq[id(i, i, 0)] = 0;
+  }
+  while (!pq.empty()) {
+    auto [c, i, j, X] = pq.top();
+    pq.pop();
+    int64_t s = id(i, j, X);
+    if (done[s] || inq[s] != c) continue;
+    done[s] = 1;
+    t.W[s] = std::min(t.W[s], c);
+    for (int32_t sub = 0; sub < X; ++sub)
+      if ((sub | X) == X) t.W[id(i, j, sub)] = std::min(t.W[id(i, j, sub)], c);
+    for (int32_t p = 0; p < k; ++p) {
+      if (X & (1 << p)) continue;
+      int32_t Xw = X | (1 << p);
+      int64_t s2 = id(i, p, Xw);
+      if (done[s2]) continue;
+      int64_t cw = c + vd[j * k + p];
+      if (inq[s2] < 0 || cw < inq[s2]) {
+        inq[s2] = cw;
+        pq.push({cw, i, p, Xw});
+      }
+    }
+  }
+  t.W2.assign((int64_t)k * t.width, INF64);
+  for (int32_t i = 0; i < k; ++i)
+    for (int32_t X = 1; X < t.width; ++X) {
+      int64_t m = INF64;
+      for (int32_t j = 0; j < k; ++j) m = std::min(m, t.w(i, j, X));
+      t.W2[(int64_t)i * t.width + X] = m;
+    }
+}
+
+// PrunedDP++ lower bound of a feasible solution containing T(v,X) of cost `cost`
+// (CPUNONHOP.h:433-493): cost + max(lb1, lb2, max one-label distance).
+int64_t lb_procedure(int32_t v, int32_t X, int64_t cost, int32_t full,
+                     const std::vector<std::vector<int32_t>>& dist, const Tours& t) {
+  if (X == full) return cost;
+  const int32_t Xs = full - X;
+  int64_t lb1 = INF64, lb2 = -1, one = -1;
+  for (int32_t i = 0; i < t.k; ++i) {
+    if (!(Xs & (1 << i))) continue;
+    int64_t dvi = dist[i][v];
+    int64_t xxx = INF64;
+    for (int32_t j = 0; j < t.k; ++j) {
+      if (!(Xs & (1 << j))) continue;
+      int64_t dvj = dist[j][v];
+      xxx = std::min(xxx, dvj);
+      lb1 = std::min(lb1, (dvi + t.w(i, j, Xs) + dvj) / 2);
+    }
+    lb2 = std::max(lb2, (dvi + t.W2[(int64_t)i * t.width + Xs] + xxx) / 2);
+    one = std::max(one, dvi);
+  }
+  int64_t lb = std::max(std::max(lb1, lb2), one);
+  return cost + lb;
+}
+
+struct EdgeOut {
+  std::vector<std::pair<int32_t, int32_t>> e;
+  void add(int32_t a, int32_t b) { e.push_back({std::min(a, b), std::max(a, b)}); }
+  void dedup() {
+    std::sort(e.begin(), e.end());
+    e.erase(std::unique(e.begin(), e.end()), e.end());
+  }
+};
+
+int write_edges(const Graph& g, EdgeOut& out, int32_t* edges_uv, int32_t cap,
+                int32_t* nedges, int32_t* cost) {
+  out.dedup();
+  int64_t sum = 0;
+  for (auto& pr : out.e) sum += g.edge_weight(pr.first, pr.second);
+  if (cost) *cost = sat(sum);
+  if (nedges) *nedges = (int32_t)out.e.size();
+  if (edges_uv) {
+    if ((int32_t)out.e.size() > cap) return CAPACITY;
+    for (size_t i = 0; i < out.e.size(); ++i) {
+      edges_uv[2 * i] = out.e[i].first;
+      edges_uv[2 * i + 1] = out.e[i].second;
+    }
+  }
+  return OK;
+}
+
+int32_t mst_weight(const Graph& g, std::vector<std::pair<int32_t, int32_t>> edges) {
+  // Kruskal over the union of a tree and shortest paths
+  // (graph_hash_of_mixed_weighted_MST_postprocessing_no_whole_graph).
+  std::sort(edges.begin(), edges.end());
+  edges.erase(std::unique(edges.begin(), edges.end()), edges.end());
+  std::vector<std::tuple<int32_t, int32_t, int32_t>> we;
+  std::vector<int32_t> verts;
+  for (auto& pr : edges) {
+    we.push_back({g.edge_weight(pr.first, pr.second), pr.first, pr.second});
+    verts.push_back(pr.first);
+    verts.push_back(pr.second);
+  }
+  std::sort(verts.begin(), verts.end());
+  verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+  std::sort(we.begin(), we.end());
+  std::vector<int32_t> par(verts.size());
+  for (size_t i = 0; i < par.size(); ++i) par[i] = (int32_t)i;
+  std::function<int32_t(int32_t)> find = [&](int32_t x) {
+    while (par[x] != x) x = par[x] = par[par[x]];
+    return x;
+  };
+  auto idx = [&](int32_t v) {
+    return (int32_t)(std::lower_bound(verts.begin(), verts.end(), v) - verts.begin());
+  };
+  int64_t sum = 0;
+  for (auto& [wt, a, b] : we) {
+    int32_t ra = find(idx(a)), rb = find(idx(b));
+    if (ra != rb) {
+      par[ra] = rb;
+      sum += wt;
+    }
+  }
+  return sat(sum);
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t orc_inf(void) { return INF; }
+
+int orc_sssp(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+             const int64_t* goff, const int32_t* gmem, int32_t group, int32_t* dist,
+             int32_t* pred) {
+  Graph g{V, beg, col, w};
+  std::vector<int32_t> src(gmem + goff[group], gmem + goff[group + 1]);
+  std::vector<int32_t> d, p;
+  dijkstra(g, src, d, p);
+  std::memcpy(dist, d.data(), sizeof(int32_t) * V);
+  if (pred) std::memcpy(pred, p.data(), sizeof(int32_t) * V);
+  return OK;
+}
+
+// Restated PrunedDP++ (CPUNONHOP.h:496-964). stats[0] = states popped,
+// stats[1] = finite states at return. dp_seconds = the big loop only (the
+// reference's time_record), total_seconds includes shortest paths and tours.
+int orc_prunedpp(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+                 int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+                 const int32_t* q, int32_t* out_cost, int32_t* edges_uv, int32_t cap,
+                 int32_t* nedges, double* dp_seconds, double* total_seconds,
+                 int64_t* stats) {
+  double t0 = now_s();
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (k >= 15 || !make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  const int32_t full = Q.full, width = 1 << k;
+  const int32_t safeguard = 2;  // error_safeguard, CPUNONHOP.h:509
+  std::vector<std::vector<int32_t>> dist, pred;
+  one_label(g, goff, gmem, k, q, dist, pred);
+  Tours tours;
+  build_tours(g, goff, gmem, k, q, dist, tours);
+
+  const int64_t NS = (int64_t)V * width;
+  std::vector<int32_t> cost(NS, INF), type(NS, -1), from(NS, -1);  // from: u or p1
+  std::vector<uint8_t> popped(NS, 0);
+  std::vector<int64_t> inq(NS, -1);
+  typedef std::tuple<int64_t, int32_t, int32_t> E;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  auto sid = [&](int32_t v, int32_t p) { return (int64_t)v * width + p; };
+  auto push = [&](int32_t v, int32_t p, int64_t prio) {
+    int64_t s = sid(v, p);
+    if (inq[s] < 0 || prio < inq[s]) {
+      inq[s] = prio;
+      pq.push({prio, v, p});
+    }
+  };
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t p = 1; p <= Q.gs[v]; ++p)
+      if ((p | Q.gs[v]) == Q.gs[v]) {
+        cost[sid(v, p)] = 0;
+        type[sid(v, p)] = 0;
+        push(v, p, 0);
+      }
+  auto build_tree = [&](int32_t rv, int32_t rp, EdgeOut& out) {
+    std::vector<std::pair<int32_t, int32_t>> st{{rv, rp}};
+    while (!st.empty()) {
+      auto [v, p] = st.back();
+      st.pop_back();
+      int64_t s = sid(v, p);
+      if (type[s] == 1) {
+        out.add(v, from[s]);
+        st.push_back({from[s], p});
+      } else if (type[s] == 2) {
+        st.push_back({v, from[s]});
+        st.push_back({v, p ^ from[s]});
+      }
+    }
+  };
+  int64_t best = INF;
+  int64_t process = 0;
+  double tloop = now_s();
+  while (!pq.empty()) {
+    auto [prio, v, X] = pq.top();
+    pq.pop();
+    int64_t s = sid(v, X);
+    if (inq[s] != prio) continue;  // superseded by a decrease-key
+    inq[s] = -1;
+    ++process;
+    const int64_t c = cost[s];
+    if (X == full) {
+      if (dp_seconds) *dp_seconds = now_s() - tloop;
+      EdgeOut out;
+      build_tree(v, X, out);
+      int rc = write_edges(g, out, edges_uv, cap, nedges, out_cost);
+      if (stats) {
+        int64_t fin = 0;
+        for (int64_t i = 0; i < NS; ++i) fin += cost[i] < INF;
+        stats[0] = process;
+        stats[1] = fin;
+      }
+      if (total_seconds) *total_seconds = now_s() - t0;
+      return rc;
+    }
+    popped[s] = 1;
+    // feasible solution: T(v,X) plus shortest paths to the uncovered groups
+    int64_t fs = c;
+    for (int32_t i = 0; i < k && fs < INF; ++i)
+      if (!(X & (1 << i))) fs = dist[i][v] >= INF ? INF : fs + dist[i][v];
+    if (fs < best) {
+      EdgeOut out;
+      build_tree(v, X, out);
+      for (int32_t i = 0; i < k; ++i) {
+        if (X & (1 << i)) continue;
+        int32_t x = v;
+        while (pred[i][x] != x) {
+          out.add(x, pred[i][x]);
+          x = pred[i][x];
+        }
+      }
+      best = mst_weight(g, out.e);
+    }
+    const int32_t Xs = full - X;
+    if (popped[sid(v, Xs)]) {  // complementary merge, CPUNONHOP.h:730-789
+      int64_t m = c + cost[sid(v, Xs)];
+      int64_t sf = sid(v, full);
+      if (m < cost[sf]) {
+        cost[sf] = sat(m);
+        type[sf] = 2;
+        from[sf] = X;
+        if (m > best + safeguard) continue;
+        if (m < best) best = m;
+        push(v, full, m);
+      }
+      continue;
+    }
+    if (c < best / 2 + safeguard) {  // Optimal-Tree Decomposition Theorem
+      for (int64_t j = beg[v]; j < beg[v + 1]; ++j) {
+        int32_t u = col[j];
+        int64_t su = sid(u, X);
+        if (popped[su]) continue;
+        int64_t gc = c + w[j];
+        if (gc < cost[su]) {
+          cost[su] = sat(gc);
+          type[su] = 1;
+          from[su] = v;
+          if (lb_procedure(u, X, gc, full, dist, tours) > best + safeguard) continue;
+          push(u, X, gc);
+        }
+      }
+      for (int32_t p2 = 1; p2 < full; ++p2) {
+        if (p2 & X) continue;
+        if (!popped[sid(v, p2)]) continue;
+        int32_t P = X | p2;
+        int64_t sP = sid(v, P);
+        if (popped[sP]) continue;
+        int64_t m = c + cost[sid(v, p2)];
+        if (m < cost[sP]) {
+          cost[sP] = sat(m);
+          type[sP] = 2;
+          from[sP] = X;
+          if ((double)m <= (double)best * 0.667 + safeguard) {  // Conditional Tree Merging
+            if (lb_procedure(v, P, m, full, dist, tours) > best + safeguard) continue;
+            if (P == full && m < best) best = m;
+            push(v, P, m);
+          }
+        }
+      }
+    }
+  }
+  if (dp_seconds) *dp_seconds = now_s() - tloop;
+  if (total_seconds) *total_seconds = now_s() - t0;
+  if (stats) {
+    stats[0] = process;
+    stats[1] = 0;
+  }
+  if (out_cost) *out_cost = INF;
+  if (nedges) *nedges = 0;
+  return INFEASIBLE;
+}
+
+// Unpruned DPBF: Dijkstra over (v, p); exact.
+int orc_dpbf(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+             int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+             const int32_t* q, int32_t* out_cost, int32_t* edges_uv, int32_t cap,
+             int32_t* nedges) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (!make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  const int32_t full = Q.full, width = 1 << k;
+  const int64_t NS = (int64_t)V * width;
+  std::vector<int32_t> cost(NS, INF), type(NS, -1), from(NS, -1);
+  std::vector<uint8_t> popped(NS, 0);
+  typedef std::tuple<int64_t, int32_t, int32_t> E;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  auto sid = [&](int32_t v, int32_t p) { return (int64_t)v * width + p; };
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t p = 1; p <= Q.gs[v]; ++p)
+      if ((p | Q.gs[v]) == Q.gs[v]) {
+        cost[sid(v, p)] = 0;
+        type[sid(v, p)] = 0;
+        pq.push({0, v, p});
+      }
+  while (!pq.empty()) {
+    auto [c, v, p] = pq.top();
+    pq.pop();
+    int64_t s = sid(v, p);
+    if (popped[s] || c != cost[s]) continue;
+    popped[s] = 1;
+    if (p == full) {
+      EdgeOut out;
+      std::vector<std::pair<int32_t, int32_t>> st{{v, p}};
+      while (!st.empty()) {
+        auto [x, y] = st.back();
+        st.pop_back();
+        int64_t t = sid(x, y);
+        if (type[t] == 1) {
+          out.add(x, from[t]);
+          st.push_back({from[t], y});
+        } else if (type[t] == 2) {
+          st.push_back({x, from[t]});
+          st.push_back({x, y ^ from[t]});
+        }
+      }
+      int32_t tree_cost = 0;
+      int rc = write_edges(g, out, edges_uv, cap, nedges, &tree_cost);
+      if (out_cost) *out_cost = (int32_t)c;
+      return rc;
+    }
+    for (int64_t j = beg[v]; j < beg[v + 1]; ++j) {
+      int32_t u = col[j];
+      int64_t nc = c + w[j];
+      int64_t su = sid(u, p);
+      if (nc < cost[su]) {
+        cost[su] = sat(nc);
+        type[su] = 1;
+        from[su] = v;
+        pq.push({nc, u, p});
+      }
+    }
+    const int32_t mask = full ^ p;
+    for (int32_t p2 = mask; p2 > 0; p2 = (p2 - 1) & mask) {
+      if (!popped[sid(v, p2)]) continue;
+      int64_t nc = c + cost[sid(v, p2)];
+      int64_t sP = sid(v, p | p2);
+      if (nc < cost[sP]) {
+        cost[sP] = sat(nc);
+        type[sP] = 2;
+        from[sP] = p;
+        pq.push({nc, v, p | p2});
+      }
+    }
+  }
+  if (out_cost) *out_cost = INF;
+  if (nedges) *nedges = 0;
+  return INFEASIBLE;
+}
+
+// Restated D-PrunedDP++ HOP_cpu (code/D-PrunedDP++/include/CPUHOP.h:107-283):
+// Dijkstra over (v, p, d) with lazy deletion, stopping at the first popped
+// full state; the tree is rebuilt from the stored predecessors.
+int orc_hop(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+            int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+            const int32_t* q, int32_t D, int32_t* out_cost, int32_t* edges_uv,
+            int32_t cap, int32_t* nedges, double* dp_seconds, int64_t* stats) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (D < 0 || D > 62 || !make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  const int32_t full = Q.full, width = 1 << k, L = D + 1;
+  const int64_t NS = (int64_t)V * width * L;
+  auto sid = [&](int32_t v, int32_t p, int32_t d) { return ((int64_t)v * width + p) * L + d; };
+  std::vector<int32_t> cost(NS, INF), type(NS, -1), a(NS, -1), b(NS, -1);
+  typedef std::tuple<int64_t, int32_t, int32_t, int32_t> E;
+  std::priority_queue<E, std::vector<E>, std::greater<E>> pq;
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t i = 0; i < k; ++i)
+      if (Q.gs[v] & (1 << i)) {
+        cost[sid(v, 1 << i, 0)] = 0;
+        type[sid(v, 1 << i, 0)] = 0;
+        pq.push({0, v, 1 << i, 0});
+      }
+  double t0 = now_s();
+  int64_t process = 0;
+  while (!pq.empty()) {
+    auto [c, v, p, d] = pq.top();
+    pq.pop();
+    if (c != cost[sid(v, p, d)]) continue;
+    ++process;
+    if (p == full) {
+      if (dp_seconds) *dp_seconds = now_s() - t0;
+      EdgeOut out;
+      std::vector<std::tuple<int32_t, int32_t, int32_t>> st{{v, p, d}};
+      while (!st.empty()) {
+        auto [x, y, z] = st.back();
+        st.pop_back();
+        int64_t t = sid(x, y, z);
+        if (type[t] == 1) {
+          out.add(x, a[t]);
+          st.push_back({a[t], y, z - 1});
+        } else if (type[t] == 2) {
+          // a = p1 | d1 << 24, b = d2 (p2 = y ^ p1)
+          int32_t p1 = a[t] & 0xFFFFFF, d1 = a[t] >> 24;
+          st.push_back({x, p1, d1});
+          st.push_back({x, y ^ p1, b[t]});
+        }
+      }
+      int32_t tree_cost = 0;
+      int rc = write_edges(g, out, edges_uv, cap, nedges, &tree_cost);
+      if (out_cost) *out_cost = (int32_t)c;
+      if (stats) {
+        int64_t fin = 0;
+        for (int64_t i = 0; i < NS; ++i) fin += cost[i] < INF;
+        stats[0] = process;
+        stats[1] = fin;
+      }
+      return rc;
+    }
+    if (d < D) {
+      for (int64_t j = beg[v]; j < beg[v + 1]; ++j) {
+        int32_t u = col[j];
+        int64_t nc = c + w[j];
+        int64_t su = sid(u, p, d + 1);
+        if (cost[su] > nc) {
+          cost[su] = sat(nc);
+          type[su] = 1;
+          a[su] = v;
+          pq.push({nc, u, p, d + 1});
+        }
+      }
+    }
+    for (int32_t p2 = 1; p2 <= full; ++p2) {
+      if (p2 & p) continue;
+      for (int32_t d2 = 0; d2 <= D - d; ++d2) {
+        int32_t P = p | p2, nd = std::max(d, d2);
+        int64_t nc = c + cost[sid(v, p2, d2)];
+        int64_t sP = sid(v, P, nd);
+        if (cost[sP] > nc) {
+          cost[sP] = sat(nc);
+          type[sP] = 2;
+          a[sP] = p | (d << 24);
+          b[sP] = d2;
+          pq.push({nc, v, P, nd});
+        }
+      }
+    }
+  }
+  if (dp_seconds) *dp_seconds = now_s() - t0;
+  if (stats) {
+    stats[0] = process;
+    stats[1] = 0;
+  }
+  if (out_cost) *out_cost = INF;
+  if (nedges) *nedges = 0;
+  return INFEASIBLE;
+}
+
+// Restated TrimCDP-multi-core-CPU cpu_mt: level-synchronous frontier over
+// `threads` workers with atomic costs; returns min(best, min_v T(v, full)).
+// stats[0] = states processed, stats[1] = frontier levels.
+int orc_trimcdp_mt(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+                   int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+                   const int32_t* q, int32_t threads, int32_t* out_cost,
+                   double* dp_seconds, double* total_seconds, int64_t* stats) {
+  double t0 = now_s();
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (k >= 15 || !make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  if (threads <= 0)
+    threads = (int32_t)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 100u);
+  const int32_t full = Q.full, width = 1 << k;
+  std::vector<std::vector<int32_t>> dist, pred;
+  one_label(g, goff, gmem, k, q, dist, pred);
+  Tours tours;
+  build_tours(g, goff, gmem, k, q, dist, tours);
+  const int64_t NS = (int64_t)V * width;
+  std::vector<std::atomic<int32_t>> cost(NS);
+  for (auto& x : cost) x.store(INF, std::memory_order_relaxed);
+  std::atomic<int32_t> best(INF);
+  auto fetch_min = [](std::atomic<int32_t>& a, int32_t v) {
+    int32_t cur = a.load(std::memory_order_relaxed);
+    while (v < cur && !a.compare_exchange_weak(cur, v, std::memory_order_relaxed)) {
+    }
+  };
+  std::vector<std::pair<int32_t, int32_t>> cur;
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t p = 1; p <= Q.gs[v]; ++p)
+      if ((p | Q.gs[v]) == Q.gs[v]) {
+        cost[(int64_t)v * width + p].store(0);
+        cur.push_back({v, p});
+      }
+  int64_t process = 0, levels = 0;
+  double tloop = now_s();
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> local(threads);
+  while (!cur.empty()) {
+    process += (int64_t)cur.size();
+    ++levels;
+    std::atomic<int64_t> next_item(0);
+    auto worker = [&](int32_t tid) {
+      auto& out = local[tid];
+      out.clear();
+      while (true) {
+        int64_t it = next_item.fetch_add(1, std::memory_order_relaxed);
+        if (it >= (int64_t)cur.size()) break;
+        const int32_t v = cur[it].first, X = cur[it].second;
+        const int32_t c = cost[(int64_t)v * width + X].load(std::memory_order_relaxed);
+        const int32_t lbest = best.load(std::memory_order_relaxed);
+        int64_t fs = c;
+        for (int32_t i = 0; i < k && fs < INF; ++i)
+          if (!(X & (1 << i))) fs = dist[i][v] >= INF ? INF : fs + dist[i][v];
+        if (fs < lbest) fetch_min(best, (int32_t)fs);
+        const int32_t Xs = full - X;
+        const int32_t cxs = cost[(int64_t)v * width + Xs].load(std::memory_order_relaxed);
+        if (cxs != INF) {
+          int32_t m = c + cxs;
+          if (m < cost[(int64_t)v * width + full].load(std::memory_order_relaxed)) {
+            fetch_min(cost[(int64_t)v * width + full], m);
+            if (m <= lbest + 2 && m < lbest) fetch_min(best, m);
+          }
+        }
+        if (c < lbest / 2 + 2) {
+          for (int64_t j = beg[v]; j < beg[v + 1]; ++j) {
+            int32_t u = col[j];
+            int32_t gc = c + w[j];
+            auto& cu = cost[(int64_t)u * width + X];
+            if (gc < cu.load(std::memory_order_relaxed)) {
+              fetch_min(cu, gc);
+              if (lb_procedure(u, X, gc, full, dist, tours) <= lbest + 2) out.push_back({u, X});
+            }
+          }
+          for (int32_t p2 = 1; p2 < full; ++p2) {
+            if (p2 & X) continue;
+            int32_t c2 = cost[(int64_t)v * width + p2].load(std::memory_order_relaxed);
+            if (c2 == INF) continue;
+            int32_t P = X + p2;
+            int32_t m = c + c2;
+            auto& cP = cost[(int64_t)v * width + P];
+            if (m < cP.load(std::memory_order_relaxed)) {
+              fetch_min(cP, m);
+              if ((double)m <= (double)lbest * 0.667 + 2) {
+                if (lb_procedure(v, P, m, full, dist, tours) <= lbest + 2) {
+                  if (P == full && m < lbest) fetch_min(best, m);
+                  out.push_back({v, P});
+                }
+              }
+            }
+          }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+    cur.clear();
+    for (auto& l : local) cur.insert(cur.end(), l.begin(), l.end());
+  }
+  int32_t fin = best.load();
+  for (int32_t v = 0; v < V; ++v) fin = std::min(fin, cost[(int64_t)v * width + full].load());
+  if (dp_seconds) *dp_seconds = now_s() - tloop;
+  if (total_seconds) *total_seconds = now_s() - t0;
+  if (stats) {
+    stats[0] = process;
+    stats[1] = levels;
+  }
+  if (out_cost) *out_cost = fin;
+  return fin >= INF ? INFEASIBLE : OK;
+}
+
+// Exhaustive plain GST on tiny graphs (V <= 20): the optimum is the minimum,
+// over vertex sets S that hit every query group and induce a connected
+// subgraph, of the minimum spanning tree weight of G[S].
+int orc_brute_plain(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+                    int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+                    const int32_t* q, int32_t* out_cost) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (V > 20 || !make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  std::vector<int32_t> adj(V * V, INF);
+  for (int32_t v = 0; v < V; ++v)
+    for (int64_t j = beg[v]; j < beg[v + 1]; ++j)
+      adj[v * V + col[j]] = std::min(adj[v * V + col[j]], w[j]);
+  std::vector<uint32_t> gm(k, 0);
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t i = 0; i < k; ++i)
+      if (Q.gs[v] & (1 << i)) gm[i] |= 1u << v;
+  int64_t best = INF;
+  std::vector<int32_t> verts, key;
+  std::vector<uint8_t> in;
+  for (uint32_t S = 1; S < (1u << V); ++S) {
+    bool hit = true;
+    for (int32_t i = 0; i < k && hit; ++i) hit = (S & gm[i]) != 0;
+    if (!hit) continue;
+    verts.clear();
+    for (int32_t v = 0; v < V; ++v)
+      if (S & (1u << v)) verts.push_back(v);
+    const int32_t n = (int32_t)verts.size();
+    key.assign(n, INF);
+    in.assign(n, 0);
+    key[0] = 0;
+    int64_t total = 0;
+    bool ok = true;
+    for (int32_t it = 0; it < n; ++it) {  // Prim
+      int32_t bi = -1;
+      for (int32_t i = 0; i < n; ++i)
+        if (!in[i] && (bi < 0 || key[i] < key[bi])) bi = i;
+      if (key[bi] >= INF) {
+        ok = false;
+        break;
+      }
+      in[bi] = 1;
+      total += key[bi];
+      if (total >= best) {
+        ok = false;
+        break;
+      }
+      for (int32_t i = 0; i < n; ++i)
+        if (!in[i]) key[i] = std::min(key[i], adj[verts[bi] * V + verts[i]]);
+    }
+    if (ok) best = std::min(best, total);
+  }
+  if (out_cost) *out_cost = (int32_t)best;
+  return best >= INF ? INFEASIBLE : OK;
+}
+
+// Exhaustive hop-bounded GST on tiny graphs (<= 22 undirected edges): minimum
+// weight over edge sets forming a tree of hop diameter <= D that touches
+// every query group (a single vertex in all groups has weight 0).
+int orc_brute_hop(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+                  int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+                  const int32_t* q, int32_t D, int32_t* out_cost) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (!make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  std::vector<std::tuple<int32_t, int32_t, int32_t>> E;
+  for (int32_t v = 0; v < V; ++v)
+    for (int64_t j = beg[v]; j < beg[v + 1]; ++j)
+      if (v < col[j]) E.push_back({v, col[j], w[j]});
+  const int32_t m = (int32_t)E.size();
+  if (m > 22) return INVALID;
+  int64_t best = INF;
+  for (int32_t v = 0; v < V; ++v)
+    if (Q.gs[v] == Q.full) best = 0;
+  std::vector<int32_t> par(V), vid(V), adjc(V);
+  std::vector<std::vector<int32_t>> adj(V);
+  std::vector<int32_t> dq(V), dd(V);
+  for (uint32_t S = 1; S < (1u << m); ++S) {
+    int64_t wt = 0;
+    int32_t ne = 0;
+    for (int32_t e = 0; e < m; ++e)
+      if (S & (1u << e)) {
+        wt += std::get<2>(E[e]);
+        ++ne;
+      }
+    if (wt >= best) continue;
+    for (int32_t v = 0; v < V; ++v) adj[v].clear();
+    int32_t nv = 0, mask = 0;
+    for (int32_t e = 0; e < m; ++e)
+      if (S & (1u << e)) {
+        auto [a, b, ww] = E[e];
+        (void)ww;
+        adj[a].push_back(b);
+        adj[b].push_back(a);
+      }
+    int32_t start = -1;
+    for (int32_t v = 0; v < V; ++v)
+      if (!adj[v].empty()) {
+        ++nv;
+        mask |= Q.gs[v];
+        start = v;
+      }
+    if (nv != ne + 1 || mask != Q.full) continue;
+    // BFS from a vertex: connectivity check, then diameter by double BFS
+    auto bfs = [&](int32_t s, int32_t& far) {
+      std::fill(dd.begin(), dd.end(), -1);
+      int32_t h = 0, t = 0, seen = 0;
+      dq[t++] = s;
+      dd[s] = 0;
+      far = s;
+      while (h < t) {
+        int32_t x = dq[h++];
+        ++seen;
+        if (dd[x] > dd[far]) far = x;
+        for (int32_t y : adj[x])
+          if (dd[y] < 0) {
+            dd[y] = dd[x] + 1;
+            dq[t++] = y;
+          }
+      }
+      return seen;
+    };
+    int32_t far = start;
+    if (bfs(start, far) != nv) continue;
+    int32_t far2 = far;
+    bfs(far, far2);
+    if (dd[far2] > D) continue;
+    best = wt;
+  }
+  if (out_cost) *out_cost = (int32_t)best;
+  return best >= INF ? INFEASIBLE : OK;
+}
+
+// Feasibility of a returned tree: every edge exists, the edges form one tree
+// (connected, acyclic), it touches every query group (a single vertex of
+// `root_hint` is accepted for zero-edge answers), and, for D >= 0, its hop
+// diameter is <= D. weight = sum of graph weights. Returns 0 when feasible,
+// otherwise 1 missing edge, 2 cycle / duplicate, 3 disconnected, 4 group not
+// touched, 5 diameter too large, 6 invalid input.
+int orc_verify_tree(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w,
+                    int32_t G, const int64_t* goff, const int32_t* gmem, int32_t k,
+                    const int32_t* q, int32_t D, int32_t nedges, const int32_t* edges_uv,
+                    int32_t root_hint, int64_t* weight) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (!make_query(g, goff, gmem, G, k, q, Q) || nedges < 0) return 6;
+  std::vector<int32_t> verts;
+  int64_t sum = 0;
+  std::vector<std::pair<int32_t, int32_t>> es;
+  for (int32_t i = 0; i < nedges; ++i) {
+    int32_t a = edges_uv[2 * i], b = edges_uv[2 * i + 1];
+    if (a < 0 || b < 0 || a >= V || b >= V) return 6;
+    int32_t ew = g.edge_weight(a, b);
+    if (ew < 0) return 1;
+    sum += ew;
+    es.push_back({std::min(a, b), std::max(a, b)});
+    verts.push_back(a);
+    verts.push_back(b);
+  }
+  if (weight) *weight = sum;
+  if (nedges == 0) {
+    if (root_hint < 0 || root_hint >= V) return 3;
+    return Q.gs[root_hint] == Q.full ? 0 : 4;
+  }
+  std::sort(es.begin(), es.end());
+  if (std::unique(es.begin(), es.end()) != es.end()) return 2;
+  std::sort(verts.begin(), verts.end());
+  verts.erase(std::unique(verts.begin(), verts.end()), verts.end());
+  if ((int64_t)verts.size() != (int64_t)nedges + 1) return (int64_t)verts.size() <= nedges ? 2 : 3;
+  auto idx = [&](int32_t v) {
+    return (int32_t)(std::lower_bound(verts.begin(), verts.end(), v) - verts.begin());
+  };
+  const int32_t n = (int32_t)verts.size();
+  std::vector<std::vector<int32_t>> adj(n);
+  for (auto& e : es) {
+    adj[idx(e.first)].push_back(idx(e.second));
+    adj[idx(e.second)].push_back(idx(e.first));
+  }
+  std::vector<int32_t> dd(n);
+  auto bfs = [&](int32_t s, int32_t& far) {
+    std::fill(dd.begin(), dd.end(), -1);
+    std::vector<int32_t> qq{s};
+    dd[s] = 0;
+    far = s;
+    for (size_t h = 0; h < qq.size(); ++h) {
+      int32_t x = qq[h];
+      if (dd[x] > dd[far]) far = x;
+      for (int32_t y : adj[x])
+        if (dd[y] < 0) {
+          dd[y] = dd[x] + 1;
+          qq.push_back(y);
+        }
+    }
+    return (int32_t)qq.size();
+  };
+  int32_t far = 0;
+  if (bfs(0, far) != n) return 3;
+  int32_t mask = 0;
+  for (int32_t v : verts) mask |= Q.gs[v];
+  if (mask != Q.full) return 4;
+  if (D >= 0) {
+    int32_t far2 = far;
+    bfs(far, far2);
+    if (dd[far2] > D) return 5;
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,172 @@
+"""GPU parity: libgst.so against the CPU oracle, bit-exact costs, verified trees.
+
+Each case prints its name, seed and shape before the GPU call.
+"""
+import numpy as np
+import pytest
+
+from gpu4gst import GSTGraph, gen, workloads, STATUS_INFEASIBLE, STATUS_OK
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _graph(g, grp):
+    G = GSTGraph.from_generated(g)
+    G.set_groups(grp)
+    return G
+
+
+def _check_tree(g, grp, q, res_cost, edges, D=-1):
+    if len(edges) == 0:
+        # a single vertex in every query group
+        gs = np.zeros(g.n, np.int64)
+        for i, gid in enumerate(q):
+            gs[grp.members(int(gid))] |= 1 << i
+        assert res_cost == 0 and (gs == (1 << len(q)) - 1).any()
+        return
+    code, wsum = O.verify_tree(g, grp, q, edges, D)
+    assert code == 0, f"tree check failed with code {code}"
+    assert wsum == res_cost
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_plain_random_small(gpu, seed):
+    rng = np.random.default_rng(1000 + seed)
+    n = int(rng.integers(4, 90))
+    m = int(rng.integers(0, 3 * n))
+    g, grp = workloads.small(n, m, 8, seed_=seed, whi=int(rng.choice([1, 3, 20, 100])),
+                             member_p=float(rng.choice([0.03, 0.1, 0.3])))
+    G = _graph(g, grp)
+    for k in (1, 2, 3, 4, 5, 6, 7):
+        q = rng.choice(8, k, replace=False).astype(np.int32)
+        _say(f"plain seed={seed} n={g.n} m={g.m} k={k} q={q.tolist()}")
+        r = G.solve(q)
+        ref = O.dpbf(g, grp, q)
+        assert r.status == STATUS_OK
+        assert r.cost == ref["cost"], (r.cost, ref["cost"])
+        _check_tree(g, grp, q, r.cost, r.edges)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_hop_random_small(gpu, seed):
+    rng = np.random.default_rng(2000 + seed)
+    n = int(rng.integers(4, 60))
+    m = int(rng.integers(0, 2 * n))
+    g, grp = workloads.small(n, m, 6, seed_=100 + seed, whi=int(rng.choice([1, 5, 50])),
+                             member_p=float(rng.choice([0.05, 0.15])))
+    G = _graph(g, grp)
+    for k in (1, 2, 3, 4, 5):
+        q = rng.choice(6, k, replace=False).astype(np.int32)
+        for D in (0, 1, 2, 3, 5):
+            _say(f"hop seed={seed} n={g.n} m={g.m} k={k} D={D} q={q.tolist()}")
+            r = G.solve(q, D=D)
+            ref = O.hop(g, grp, q, D)
+            if ref["status"] == O.INFEASIBLE:
+                assert r.status == STATUS_INFEASIBLE
+                continue
+            assert r.cost == ref["cost"], (r.cost, ref["cost"])
+            _check_tree(g, grp, q, r.cost, r.edges, D)
+
+
+def test_hop_prune_off_matches(gpu):
+    g, grp = workloads.small(50, 90, 6, seed_=5, whi=40)
+    G = _graph(g, grp)
+    q = np.array([0, 1, 2, 3], np.int32)
+    _say("hop prune on/off n=50 k=4 D=3")
+    a = G.solve(q, D=3).cost
+    G.set_option("hop_prune", 0)
+    b = G.solve(q, D=3).cost
+    assert a == b == O.hop(g, grp, q, 3)["cost"]
+
+
+def test_batch_equals_single(gpu):
+    g, grp = workloads.small(120, 300, 10, seed_=11, whi=50, member_p=0.05)
+    G = _graph(g, grp)
+    rng = np.random.default_rng(3)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(1, 8, 40)]
+    _say("batch of 40 plain queries n=120")
+    br = G.solve_batch(qs)
+    for i, q in enumerate(qs):
+        ref = O.dpbf(g, grp, q)
+        if ref["status"] == O.INFEASIBLE:
+            assert br.status[i] == STATUS_INFEASIBLE
+            continue
+        assert br.costs[i] == ref["cost"], (i, br.costs[i], ref["cost"])
+        _check_tree(g, grp, q, br.costs[i], br.tree(i))
+    Ds = rng.integers(1, 5, len(qs)).astype(np.int32)
+    _say("batch of 40 hop queries n=120")
+    bh = G.solve_batch(qs, D=Ds)
+    for i, q in enumerate(qs):
+        ref = O.hop(g, grp, q, int(Ds[i]))
+        if ref["status"] == O.INFEASIBLE:
+            assert bh.status[i] == STATUS_INFEASIBLE
+            continue
+        assert bh.costs[i] == ref["cost"], (i, bh.costs[i], ref["cost"])
+        _check_tree(g, grp, q, bh.costs[i], bh.tree(i), int(Ds[i]))
+
+
+def test_infeasible_and_trivial(gpu):
+    # two components: {0,1,2} and {3,4}
+    g = gen.from_edges(5, [0, 1, 3], [1, 2, 4], [4, 5, 6])
+    grp = gen.groups_from_lists([[0], [2], [4], [1, 2], [0, 1, 2]])
+    G = _graph(g, grp)
+    _say("infeasible / trivial cases")
+    r = G.solve(np.array([0, 2], np.int32))
+    assert r.status == STATUS_INFEASIBLE and len(r.edges) == 0
+    r = G.solve(np.array([0, 1], np.int32))
+    assert r.status == STATUS_OK and r.cost == 9
+    r = G.solve(np.array([3], np.int32))
+    assert r.cost == 0 and len(r.edges) == 0
+    r = G.solve(np.array([1, 3, 4], np.int32))  # vertex 2 is in groups 1, 3 and 4
+    assert r.cost == 0
+    r = G.solve(np.array([0, 1], np.int32), D=1)
+    assert r.status == STATUS_INFEASIBLE
+    r = G.solve(np.array([0, 1], np.int32), D=2)
+    assert r.cost == 9
+    G.set_option("infeasible_cost", 100000)
+    assert G.solve(np.array([0, 2], np.int32)).cost == 100000
+
+
+def test_rmat15_queries_vs_prunedpp(gpu):
+    w = workloads.config(1, n_queries=24)
+    G = _graph(w.graph, w.groups)
+    _say(f"rmat15 n={w.graph.n} m={w.graph.m} 24 queries k=3 (batch)")
+    br = G.solve_batch(w.queries)
+    for i, q in enumerate(w.queries):
+        ref = O.prunedpp(w.graph, w.groups, q)
+        assert br.costs[i] == ref["cost"], (i, br.costs[i], ref["cost"])
+        _check_tree(w.graph, w.groups, q, br.costs[i], br.tree(i))
+    for i in range(4):
+        _say(f"rmat15 single query {i}")
+        r = G.solve(w.queries[i])
+        assert r.cost == br.costs[i]
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_hub_rows_split_into_chunks(gpu, seed):
+    """Rows longer than one chunk are queued as several items (virtual split)."""
+    rng = np.random.default_rng(900 + seed)
+    g = gen.chung_lu(400, 2000, 2.1, 200.0, seed=seed, wlo=1, whi=int(rng.choice([5, 50])))
+    grp = gen.groups_zipf(g.n, 12, 1.2, 1, 30, seed=seed)
+    G = _graph(g, grp)
+    G.set_option("chunk", 64)
+    qs = [rng.choice(12, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 8)]
+    _say(f"hub graph seed={seed} n={g.n} maxdeg={int(np.diff(g.beg).max())} chunk=64")
+    br = G.solve_batch(qs)
+    Ds = rng.integers(1, 5, len(qs)).astype(np.int32)
+    bh = G.solve_batch(qs, D=Ds)
+    for i, q in enumerate(qs):
+        ref = O.dpbf(g, grp, q)
+        assert br.costs[i] == ref["cost"]
+        _check_tree(g, grp, q, br.costs[i], br.tree(i))
+        rh = O.hop(g, grp, q, int(Ds[i]))
+        if rh["status"] == O.INFEASIBLE:
+            assert bh.status[i] == STATUS_INFEASIBLE
+        else:
+            assert bh.costs[i] == rh["cost"]
+            _check_tree(g, grp, q, bh.costs[i], bh.tree(i), int(Ds[i]))
