@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--sssp-delta", type=int, default=-1)
+    ap.add_argument("--dp-div", type=int, default=-1)
+    ap.add_argument("--grid", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -81,6 +84,12 @@ def main():
     G.set_groups(w.groups)
     if args.chunk:
         G.set_option("chunk", args.chunk)
+    if args.sssp_delta >= 0:
+        G.set_option("sssp_delta", args.sssp_delta)
+    if args.dp_div >= 0:
+        G.set_option("dp_delta_div", args.dp_div)
+    if args.grid:
+        G.set_option("grid", args.grid)
     torch.cuda.synchronize()
 
     for i in range(args.warmup):

@@ -27,6 +27,7 @@ struct dim3 {
 struct int4 {
   int x, y, z, w;
 };
+inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 
 #define __global__
 #define __device__

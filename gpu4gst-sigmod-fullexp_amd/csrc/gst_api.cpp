@@ -101,8 +101,11 @@ struct gst_graph {
   int32_t infeasible_cost = INT32_MAX;
   int32_t max_tree_edges = 4096;
   int32_t use_graphs = 1;
+  int32_t sssp_delta = 0;  // 0 = max(1, largest weight / 8)
+  int32_t dp_div = 16;
+  int32_t maxw = 1;
   // engine buffers (grow-only)
-  DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr;
+  DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
   DBuf out_uv, out_eid, out_n, out_err;
   uint32_t* h_cnt = nullptr;  // pinned
   uint32_t wl_cap = 0;
@@ -166,6 +169,10 @@ DevCtx make_ctx(gst_graph* g) {
   c.iter = g->iter.as<int32_t>();
   c.ctr = g->ctr.as<unsigned long long>();
   c.cap = g->wl_cap;
+  c.sched = g->sched.as<Sched>();
+  c.nslots = g->iter.as<int32_t>() + 1;
+  c.sssp_delta = g->sssp_delta > 0 ? g->sssp_delta : std::max(1, g->maxw / 8);
+  c.dp_div = g->dp_div;
   return c;
 }
 
@@ -300,6 +307,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(g->VVb.ensure((size_t)nslots * VV * 4, &grew));
   HIP_TRY(g->bestw.ensure((size_t)nslots * 8, &grew));
   HIP_TRY(g->slots.ensure((size_t)nslots * sizeof(SlotDesc), &grew));
+  HIP_TRY(g->sched.ensure((size_t)nslots * sizeof(Sched), &grew));
   HIP_TRY(g->tasks.ensure(tasks.size() * sizeof(Task) + 16, &grew));
   if ((uint64_t)cap * 8 > g->wl0.bytes || (uint64_t)cap * 8 > g->wl1.bytes) {
     HIP_TRY(g->wl0.ensure((size_t)cap * 8, &grew));
@@ -325,6 +333,20 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(hipMemcpyAsync(g->tasks.p, tasks.data(), tasks.size() * sizeof(Task),
                          hipMemcpyHostToDevice, g->stream));
   DevCtx c = make_ctx(g);
+  {
+    std::vector<Sched> sch(nslots);
+    for (auto& x : sch) {
+      std::memset(&x, 0, sizeof(x));
+      x.thr = x.delta = c.sssp_delta;
+      x.mindef = INF;
+      x.R = INF;
+    }
+    HIP_TRY(hipMemcpyAsync(g->sched.p, sch.data(), nslots * sizeof(Sched), hipMemcpyHostToDevice,
+                           g->stream));
+    HIP_TRY(hipMemcpyAsync(g->iter.as<int32_t>() + 1, &nslots, sizeof(int32_t),
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
   launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
   launch_init_slot_tables(c, nslots, t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
@@ -338,10 +360,12 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   int64_t dp_launches = 0, sssp_launches = 0;
   int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
   if (rc) return rc;  // includes 100 = overflow
+  launch_clamp(c, nslots, g->stream);
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
   launch_route(c, nslots, g->stream);
   launch_dp_seed(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(),
                  cnt, g->stream);
+  launch_dp_sched_init(c, nslots, g->stream);
   HIP_TRY(hipGetLastError());
   rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
   if (rc) return rc;
@@ -461,6 +485,7 @@ int upload_graph(gst_graph* g, const I* beg, const I* csr, const I* wgt, int64_t
       if (ww < 1 || ww > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
       col[j] = (int32_t)u;
       w[j] = (int32_t)ww;
+      if (ww > g->maxw) g->maxw = (int32_t)ww;
       f_out += mix64((uint64_t)v * 0x9E3779B97F4A7C15ull ^ (uint64_t)u * 0xC2B2AE3D27D4EB4Full ^
                      (uint64_t)ww * 0x165667B19E3779F9ull);
       f_in += mix64((uint64_t)u * 0x9E3779B97F4A7C15ull ^ (uint64_t)v * 0xC2B2AE3D27D4EB4Full ^
@@ -613,6 +638,14 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
     case GST_OPT_USE_GRAPHS:
       g->use_graphs = v ? 1 : 0;
       break;
+    case GST_OPT_SSSP_DELTA:
+      if (v < 0 || v > INF / 2) return fail(GST_ERR_INVALID, "band width out of range");
+      g->sssp_delta = (int32_t)v;
+      break;
+    case GST_OPT_DP_DELTA_DIV:
+      if (v < 0 || v > (1 << 20)) return fail(GST_ERR_INVALID, "band divisor out of range");
+      g->dp_div = (int32_t)v;
+      break;
     default:
       return fail(GST_ERR_INVALID, "unknown option");
   }
@@ -723,7 +756,7 @@ void gst_graph_destroy(gst_graph* g) {
   g->drop_graphs();
   DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
-                  &g->cnt, &g->iter, &g->ctr, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
+                  &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();
   if (g->h_cnt) (void)hipHostFree(g->h_cnt);
   if (g->ev0) (void)hipEventDestroy(g->ev0);

@@ -77,6 +77,17 @@ enum Ctr {
   C_NUM
 };
 
+// Per-slot frontier ordering (bucketed label correcting): a state is
+// expanded only when its cost is below the slot's threshold `thr`; costlier
+// states are carried to the next iteration. When an iteration expands
+// nothing for a slot, thr jumps to the cheapest carried cost + delta.
+// R bounds the one-label phase: distances >= R are clamped to R afterwards
+// (still valid lower bounds) and never enter upper bounds.
+struct Sched {
+  int32_t thr, mindef, active, delta;
+  int32_t R, pad[3];
+};
+
 struct DevCtx {
   const int32_t* rowp;  // [V+1]
   const int32_t* col;   // [E]
@@ -95,6 +106,10 @@ struct DevCtx {
   int32_t* iter;                 // current iteration number (stamps use iter + 1)
   unsigned long long* ctr;       // Ctr counters
   uint32_t cap;                  // worklist capacity (items)
+  Sched* sched;                  // [slots]
+  const int32_t* nslots;         // number of slots of the current batch (device)
+  int32_t sssp_delta;            // band width of the one-label phase
+  int32_t dp_div;                // DP band width = max(1, best / dp_div); 0 = unordered
 };
 
 struct WL {
@@ -128,5 +143,7 @@ void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float
                      float* gw, int64_t nund, hipStream_t st);
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
                              hipStream_t st);
+void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
+void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
 
 }  // namespace gst

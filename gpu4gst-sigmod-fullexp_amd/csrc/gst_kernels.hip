@@ -85,10 +85,11 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
 // and the two tour bounds of PrunedDP++ (code/PrunedDP++/include/
 // CPUNONHOP.h:433-493) over exact group-route tables R1[i][j] = cheapest
 // group sequence i ... j visiting all of U, R2[i] = min_j R1[i][j].
-// Also returns sum_{i in U} c_i(u) (capped at INF) for the one-label upper
-// bound. Returns INF when some group of U is unreachable from u.
+// Also returns sum_{i in U} c_i(u) for the one-label upper bound (INF when
+// some distance is a clamped value >= radius). Returns INF when some group of
+// U is unreachable from u.
 __device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_t U,
-                                  int32_t* sumc) {
+                                  int32_t radius, int32_t* sumc) {
   const int4 a = *reinterpret_cast<const int4*>(crow);
   const int4 b = *reinterpret_cast<const int4*>(crow + 4);
   const int32_t cs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -105,7 +106,8 @@ __device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_
     *sumc = INF;
     return INF;
   }
-  *sumc = sum >= INF ? INF : (int32_t)sum;
+  // distances >= radius are clamped lower bounds, not path lengths
+  *sumc = (sum >= INF || one >= radius) ? INF : (int32_t)sum;
   int32_t lb1 = INF, lb2 = 0;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -221,11 +223,9 @@ __global__ __launch_bounds__(BLOCK) void k_groups_setup(DevCtx c, const Task* ta
         v = (uint32_t)gmem[m];
         atomicOr(&c.T[S.t_off + v], 1 << tk.bit);
         atomicMin(&c.C[S.c_off + (int64_t)v * CROW + tk.bit], 0);
-        int32_t* st = &c.E[S.e_off + (int64_t)tk.bit * c.Vp + v];
-        if (atomicMax(st, stamp) < stamp) want = nchunks(c, v);
+        if (atomicMax(&c.E[S.e_off + v], stamp) < stamp) want = nchunks(c, v);
       }
-      wave_append(want, mk_item(v, (uint32_t)tk.bit, 0, 0, (uint32_t)tk.slot), wl, cnt, c.cap,
-                  c.ctr);
+      wave_append(want, mk_item(v, 0, 0, 0, (uint32_t)tk.slot), wl, cnt, c.cap, c.ctr);
     }
   }
 }
@@ -238,17 +238,20 @@ void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const i
 }
 
 // ---------------------------------------------------------------------------
-// one-label phase: label-correcting multi-source shortest paths, all groups of
-// all slots in one worklist. Item (v, i): relax c_i over v's CSR chunk.
+// one-label phase: label-correcting multi-source shortest paths for all k
+// groups of a slot at once. An item is (slot, v); its k distances sit in one
+// 32-byte row c[v][0..7], so one edge visit relaxes all k labels with a
+// single row access at u. Plain queries drop labels at or beyond the running
+// upper bound + 2 (see k_clamp).
 
 __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
-  __shared__ int32_t s_cost[BLOCK];
-  __shared__ int64_t s_cbase[BLOCK];   // C offset of column i for the item's slot
-  __shared__ int64_t s_ebase[BLOCK];   // stamp row base
-  __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | i
+  __shared__ int4 s_cv[BLOCK][2];      // the item's distances (masked to INF above the radius)
+  __shared__ int64_t s_cbase[BLOCK];   // C offset of the item's slot
+  __shared__ int64_t s_ebase[BLOCK];   // stamp row of the item's slot
+  __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | k << 1 | plain
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
@@ -259,19 +262,32 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
     int32_t deg = 0;
     if (idx < n) {
       const uint64_t it = w.in[idx];
-      const uint32_t v = it_v(it), i = it_p(it), ch = it_ch(it), s = it_s(it);
+      const uint32_t v = it_v(it), ch = it_ch(it), s = it_s(it);
       const SlotDesc S = c.slots[s];
-      const int64_t cb = S.c_off + i;
-      const int32_t t = c.C[cb + (int64_t)v * CROW];
-      const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
-      const int32_t re = min(c.rowp[v + 1], rb + CH);
-      deg = t < INF ? max(0, re - rb) : 0;
-      s_beg[tid] = rb;
-      s_cost[tid] = t;
-      s_cbase[tid] = cb;
-      s_ebase[tid] = S.e_off + (int64_t)i * c.Vp;
-      s_meta[tid] = (s << 8) | i;
-      ++n_items;
+      const int32_t best = S.D < 0 ? best_of(&c.bestw[s]) : INF;
+      const int32_t radius = best < INF - 2 ? best + 2 : INF;
+      const int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
+      int4 a = *reinterpret_cast<const int4*>(row);
+      int4 b = *reinterpret_cast<const int4*>(row + 4);
+      int32_t cv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      bool live = false;
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        if (i >= S.k || cv[i] >= radius) cv[i] = INF;
+        live = live || cv[i] < INF;
+      }
+      if (live) {
+        const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
+        const int32_t re = min(c.rowp[v + 1], rb + CH);
+        deg = max(0, re - rb);
+        s_beg[tid] = rb;
+        s_cv[tid][0] = make_int4(cv[0], cv[1], cv[2], cv[3]);
+        s_cv[tid][1] = make_int4(cv[4], cv[5], cv[6], cv[7]);
+        s_cbase[tid] = S.c_off;
+        s_ebase[tid] = S.e_off;
+        s_meta[tid] = (s << 8) | ((uint32_t)S.k << 1) | (S.D < 0 ? 1u : 0u);
+        ++n_items;
+      }
     }
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
@@ -283,14 +299,41 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         const int j = find_item(s_off, e);
         const int32_t ej = s_beg[j] + (e - s_off[j]);
         const uint32_t u = (uint32_t)c.col[ej];
-        const int32_t cost = s_cost[j] + c.wt[ej];
-        int32_t* dst = &c.C[s_cbase[j] + (int64_t)u * CROW];
+        const int32_t wgt = c.wt[ej];
+        const uint32_t meta = s_meta[j], sl = meta >> 8, k = (meta >> 1) & 7u;
+        const int4 va = s_cv[j][0], vb = s_cv[j][1];
+        const int32_t cv[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+        int32_t* row = &c.C[s_cbase[j] + (int64_t)u * CROW];
+        const int4 a = *reinterpret_cast<const int4*>(row);
+        const int4 b = *reinterpret_cast<const int4*>(row + 4);
+        int32_t cu[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
         ++n_relax;
-        if (cost < *dst && cost < atomicMin(dst, cost)) {
-          int32_t* st = &c.E[s_ebase[j] + u];
-          if (atomicMax(st, next) < next) {
+        bool improved = false;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
+          if (cv[i] < INF && cand < cu[i]) {
+            const int32_t old = atomicMin(&row[i], cand);
+            if (cand < old) improved = true;
+            cu[i] = min(cand, old);
+          }
+        }
+        if (improved) {
+          if (meta & 1u) {
+            // every finite distance is a real path length: u joined to all groups
+            int64_t sum = 0;
+            bool fin = true;
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+              if (i < (int)k) {
+                fin = fin && cu[i] < INF;
+                sum += cu[i];
+              }
+            if (fin && sum < INF) ub_update(&c.bestw[sl], (int32_t)sum, wit_onelabel(u, 0));
+          }
+          if (atomicMax(&c.E[s_ebase[j] + u], next) < next) {
             cnt = nchunks(c, u);
-            item = mk_item(u, s_meta[j] & 0xFFu, 0, 0, s_meta[j] >> 8);
+            item = mk_item(u, 0, 0, 0, sl);
           }
         }
       }
@@ -307,9 +350,18 @@ void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st) 
 }
 
 // Consumes one iteration: clears the consumed list's counter, advances the
-// iteration number and records frontier statistics.
+// iteration number, moves each slot's threshold past an exhausted band and
+// records frontier statistics.
 __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_next, int dp) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    Sched& x = c.sched[s];
+    if (!x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
+    x.active = 0;
+    x.mindef = INF;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
     const uint32_t produced = *cnt_next;
     if (*cnt_consumed > 0) atomicAdd(&c.ctr[dp ? C_DP_ITERS : C_SSSP_ITERS], 1ull);
     if (dp && produced > c.ctr[C_MAX_FRONTIER]) c.ctr[C_MAX_FRONTIER] = produced;
@@ -320,7 +372,55 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
 
 void launch_advance(const DevCtx& c, uint32_t* cnt_consumed, const uint32_t* cnt_next, bool dp,
                     hipStream_t st) {
-  hipLaunchKernelGGL(k_advance, 1, 64, 0, st, c, cnt_consumed, cnt_next, dp ? 1 : 0);
+  hipLaunchKernelGGL(k_advance, 1, 256, 0, st, c, cnt_consumed, cnt_next, dp ? 1 : 0);
+}
+
+// After the one-label phase of a plain query: every vertex whose distance to
+// group i is below R = best + 2 holds it exactly (states are expanded in
+// cost bands and none below R was dropped); every other distance is >= R, so
+// R is a valid lower bound for it.
+__global__ void k_clamp(DevCtx c) {
+  const int s = blockIdx.y;
+  const SlotDesc S = c.slots[s];
+  int32_t R = INF;
+  if (S.D < 0) {
+    const int32_t best = best_of(&c.bestw[s]);
+    R = best < INF - 2 ? best + 2 : INF;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) c.sched[s].R = R;
+  if (R >= INF) return;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.V; v += gridDim.x * blockDim.x) {
+    int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
+    for (int i = 0; i < S.k; ++i)
+      if (row[i] > R) row[i] = R;
+  }
+}
+
+void launch_clamp(const DevCtx& c, int nslots, hipStream_t st) {
+  if (nslots <= 0) return;
+  dim3 grid((c.V + 255) / 256 < 1024 ? (c.V + 255) / 256 : 1024, nslots);
+  hipLaunchKernelGGL(k_clamp, grid, 256, 0, st, c);
+}
+
+// DP band width from the upper bound known after the one-label phase.
+__global__ void k_dp_sched_init(DevCtx c) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    const SlotDesc S = c.slots[s];
+    const int32_t best = best_of(&c.bestw[s]);
+    int32_t delta = c.sssp_delta;
+    if (c.dp_div <= 0) delta = INF;
+    else if (S.D < 0 && best < INF) delta = max(1, best / c.dp_div);
+    Sched& x = c.sched[s];
+    x.delta = delta;
+    x.thr = delta;
+    x.active = 0;
+    x.mindef = INF;
+  }
+}
+
+void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_sched_init, 1, 256, 0, st, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -440,7 +540,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
               int32_t sumc;
               const uint32_t U = (uint32_t)S.full ^ p;
               (void)lb_eval(&c.C[S.c_off + (int64_t)v * CROW], &c.R[S.r_off + U * RSTRIDE], U,
-                            &sumc);
+                            c.sched[tk.slot].R, &sumc);
               if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
             }
             if (atomicMax(&c.E[S.e_off + (int64_t)p * c.Vp + v], stamp) < stamp) {
@@ -502,6 +602,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
     int32_t deg = 0;
     uint32_t v = 0, p = 0, d = 0, s = 0, mask = 0;
     int32_t t = INF, Dq = 0;
+    uint32_t carry_cnt = 0;
+    uint64_t citem = 0;
     SlotDesc S;
     S.t_off = S.e_off = S.c_off = S.r_off = 0;
     S.k = S.full = S.D = S.nsub = S.vv_off = 0;
@@ -517,9 +619,12 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       Dq = S.D;
       const int64_t row = HOP ? (int64_t)d * S.nsub + p : (int64_t)p;
       t = c.T[S.t_off + row * Vp + v];
+      Sched* sc = &c.sched[s];
       bool expand = t < INF;
       if (!HOP) {
-        if (ch == 0) {
+        const int32_t best = best_of(&c.bestw[s]);
+        const bool half = t < best / 2 + 2;  // 1/2 rule: costlier states are never expanded
+        if (ch == 0 && (!half || t < sc->thr)) {
           const uint32_t xs = (uint32_t)S.full ^ p;
           const int32_t t2 = c.T[S.t_off + (int64_t)xs * Vp + v];
           if (t2 < INF && t < INF) {
@@ -529,10 +634,18 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               ub_update(&c.bestw[s], mcost, wit_full(v, 0));
           }
         }
-        const int32_t best = best_of(&c.bestw[s]);
-        expand = expand && t < best / 2 + 2;
+        if (!half) expand = false;
+      }
+      if (expand && t >= sc->thr) {  // not in the current cost band yet
+        expand = false;
+        if (t < sc->mindef) atomicMin(&sc->mindef, t);
+        if (atomicMax(&c.E[S.e_off + row * Vp + v], next) < next) {
+          carry_cnt = nchunks(c, v);
+          citem = mk_item(v, p, d, 0, s);
+        }
       }
       if (expand) {
+        if (!sc->active) sc->active = 1;
         nF += ch == 0;
         const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
         const int32_t re = min(c.rowp[v + 1], rb + CH);
@@ -544,6 +657,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         if (ch == 0) mask = (uint32_t)S.full ^ p;
       }
     }
+
+    wave_append(carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr);
 
     // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
     {
@@ -574,7 +689,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                 if (!HOP || c.hop_prune) {
                   int32_t sumc;
                   const int32_t lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW],
-                                             &c.R[S.r_off + U * RSTRIDE], U, &sumc);
+                                             &c.R[S.r_off + U * RSTRIDE], U, c.sched[s].R, &sumc);
                   nLb += __popc(U);
                   int32_t best = best_of(&c.bestw[s]);
                   if (!HOP && sumc < INF && mc + sumc < best) {
@@ -632,7 +747,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           if (!HOP || c.hop_prune) {
             int32_t sumc;
             const int32_t lb = lb_eval(&c.C[Sj.c_off + (int64_t)u * CROW],
-                                       &c.R[Sj.r_off + U * RSTRIDE], U, &sumc);
+                                       &c.R[Sj.r_off + U * RSTRIDE], U, c.sched[sl].R, &sumc);
             nLb += __popc(U);
             int32_t best = best_of(&c.bestw[sl]);
             if (!HOP && sumc < INF && gc + sumc < best) {

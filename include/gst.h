@@ -91,7 +91,9 @@ typedef struct gst_stats {
                                       100000 reproduces the reference's inf) */
 #define GST_OPT_MAX_TREE_EDGES 7   /* per-query edge capacity of the backtracking kernel */
 #define GST_OPT_USE_GRAPHS 8       /* 1 = capture iteration steps in HIP graphs (default 1) */
-#define GST_OPT_BLOCK_ITEMS 9      /* reserved */
+#define GST_OPT_SSSP_DELTA 10      /* cost band of the one-label phase (0 = largest weight / 8;
+                                      a huge value = unordered label correcting) */
+#define GST_OPT_DP_DELTA_DIV 11    /* DP cost band = upper bound / value (default 16; 0 = unordered) */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
  * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge
