@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--sssp-delta", type=int, default=-1)
     ap.add_argument("--dp-div", type=int, default=-1)
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--single-chunk", type=int, default=0,
+                    help="virtual-split chunk for the single-query path (0 = same as batched)")
+    ap.add_argument("--single-dp-div", type=int, default=-1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -109,8 +112,12 @@ def main():
     st = base.stats
     log(f"batched: {qps:.1f} queries/s, {ms_step:.1f} ms/step, stats {st}")
 
-    # single-query path on a sample (one query per call, same GPU)
+    # single-query path on a sample (one query per call, same GPU), with its own tuning
     ns = min(args.single_sample, len(w.queries))
+    if args.single_chunk:
+        G.set_option("chunk", args.single_chunk)
+    if args.single_dp_div >= 0:
+        G.set_option("dp_delta_div", args.single_dp_div)
     G.solve(w.queries[0])
     torch.cuda.synchronize()
     t1 = time.perf_counter()
@@ -123,6 +130,10 @@ def main():
     mism_single = sum(int(base.costs[i] != single[i]) for i in range(ns))
     log(f"single: {single_qps:.1f} queries/s over {ns}; mismatches {mism_single}")
 
+    if args.single_chunk:
+        G.set_option("chunk", args.chunk or 4096)
+    if args.single_dp_div >= 0:
+        G.set_option("dp_delta_div", args.dp_div if args.dp_div >= 0 else 0)
     # instrumented pass: DP expand kernel time with HIP events on the solver's stream
     G.set_option("profile", 1)
     pr = G.solve_batch(w.queries)

@@ -86,6 +86,8 @@ struct gst_graph {
   int64_t n_und = 0;
   int32_t cus = 256;
   std::vector<int32_t> h_rowp;
+  // host copies in the caller's CSR order; the device rows are sorted by weight
+  std::vector<int32_t> h_col, h_w, h_ids, perm;  // perm[device position] = caller's CSR index
   DBuf rowp, col, wt, eid;
   bool has_eid = false;
   // groups
@@ -467,6 +469,44 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
   return GST_OK;
 }
 
+// Device rows are sorted by (weight, neighbour): an expansion whose useful
+// edges are those with weight <= some limit scans only a prefix of the row.
+// Edge ids follow the entries (the caller's ids, or the caller's CSR index).
+int upload_rows(gst_graph* g) {
+  const int64_t E = g->E;
+  g->perm.resize(E);
+  for (int32_t v = 0; v < g->V; ++v) {
+    const int32_t b = g->h_rowp[v], e = g->h_rowp[v + 1];
+    for (int32_t j = b; j < e; ++j) g->perm[j] = j;
+    std::sort(g->perm.begin() + b, g->perm.begin() + e, [&](int32_t x, int32_t y) {
+      if (g->h_w[x] != g->h_w[y]) return g->h_w[x] < g->h_w[y];
+      if (g->h_col[x] != g->h_col[y]) return g->h_col[x] < g->h_col[y];
+      return x < y;
+    });
+  }
+  std::vector<int32_t> col(E), w(E), id(E);
+  g->maxw = 1;
+  for (int64_t j = 0; j < E; ++j) {
+    const int32_t o = g->perm[j];
+    col[j] = g->h_col[o];
+    w[j] = g->h_w[o];
+    id[j] = g->h_ids.empty() ? o : g->h_ids[o];
+    g->maxw = std::max(g->maxw, w[j]);
+  }
+  HIP_TRY(g->col.ensure((size_t)std::max<int64_t>(E, 1) * 4));
+  HIP_TRY(g->wt.ensure((size_t)std::max<int64_t>(E, 1) * 4));
+  HIP_TRY(g->eid.ensure((size_t)std::max<int64_t>(E, 1) * 4));
+  if (E) {
+    HIP_TRY(hipMemcpy(g->col.p, col.data(), E * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->wt.p, w.data(), E * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->eid.p, id.data(), E * 4, hipMemcpyHostToDevice));
+  }
+  g->has_eid = true;
+  ++g->gen;
+  g->drop_graphs();
+  return GST_OK;
+}
+
 template <class I>
 int upload_graph(gst_graph* g, const I* beg, const I* csr, const I* wgt, int64_t V, int64_t E) {
   if (V < 1 || V >= (int64_t)MAX_V) return fail(GST_ERR_INVALID, "V out of range (1 .. 2^24-1)");
@@ -499,20 +539,17 @@ int upload_graph(gst_graph* g, const I* beg, const I* csr, const I* wgt, int64_t
   g->V = (int32_t)V;
   g->Vp = (int32_t)((V + 63) / 64 * 64);
   g->E = E;
-  HIP_TRY(g->rowp.ensure((size_t)(V + 1) * 4));
-  HIP_TRY(g->col.ensure((size_t)std::max<int64_t>(E, 1) * 4));
-  HIP_TRY(g->wt.ensure((size_t)std::max<int64_t>(E, 1) * 4));
-  HIP_TRY(hipMemcpy(g->rowp.p, g->h_rowp.data(), (V + 1) * 4, hipMemcpyHostToDevice));
-  if (E) {
-    HIP_TRY(hipMemcpy(g->col.p, col.data(), E * 4, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(g->wt.p, w.data(), E * 4, hipMemcpyHostToDevice));
-  }
   for (int64_t v = 0; v < V; ++v) {
     const int64_t deg = (int64_t)beg[v + 1] - (int64_t)beg[v];
     if (deg > ((int64_t)MAX_CHUNKS << 6))
       return fail(GST_ERR_INVALID, "vertex degree above 2^22 is not supported");
   }
-  return GST_OK;
+  g->h_col = std::move(col);
+  g->h_w = std::move(w);
+  g->n_und = E;
+  HIP_TRY(g->rowp.ensure((size_t)(V + 1) * 4));
+  HIP_TRY(hipMemcpy(g->rowp.p, g->h_rowp.data(), (V + 1) * 4, hipMemcpyHostToDevice));
+  return upload_rows(g);
 }
 
 }  // namespace
@@ -572,20 +609,28 @@ int gst_graph_set_edge_ids(gst_graph* g, const int32_t* ids, int64_t num_undirec
   if (!g || !ids || num_undirected < 0) return fail(GST_ERR_INVALID, "bad argument");
   for (int64_t j = 0; j < g->E; ++j)
     if (ids[j] < 0 || ids[j] >= num_undirected) return fail(GST_ERR_INVALID, "edge id out of range");
-  HIP_TRY(g->eid.ensure((size_t)std::max<int64_t>(g->E, 1) * 4));
-  if (g->E) HIP_TRY(hipMemcpy(g->eid.p, ids, g->E * 4, hipMemcpyHostToDevice));
-  g->has_eid = true;
+  g->h_ids.assign(ids, ids + g->E);
   g->n_und = num_undirected;
-  ++g->gen;
-  return GST_OK;
+  return upload_rows(g);
 }
 
 int gst_graph_set_weights(gst_graph* g, const int32_t* w) {
   if (!g || !w) return fail(GST_ERR_INVALID, "bad argument");
   for (int64_t j = 0; j < g->E; ++j)
     if (w[j] < 1 || w[j] > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
-  if (g->E) HIP_TRY(hipMemcpy(g->wt.p, w, g->E * 4, hipMemcpyHostToDevice));
-  return GST_OK;
+  // weights must stay symmetric: entry (v,u) and (u,v) carry the same value
+  uint64_t f_out = 0, f_in = 0;
+  for (int32_t v = 0; v < g->V; ++v)
+    for (int32_t j = g->h_rowp[v]; j < g->h_rowp[v + 1]; ++j) {
+      const uint64_t u = (uint64_t)g->h_col[j], ww = (uint64_t)w[j];
+      f_out += mix64((uint64_t)v * 0x9E3779B97F4A7C15ull ^ u * 0xC2B2AE3D27D4EB4Full ^
+                     ww * 0x165667B19E3779F9ull);
+      f_in += mix64(u * 0x9E3779B97F4A7C15ull ^ (uint64_t)v * 0xC2B2AE3D27D4EB4Full ^
+                    ww * 0x165667B19E3779F9ull);
+    }
+  if (f_out != f_in) return fail(GST_ERR_INVALID, "weights are not symmetric");
+  g->h_w.assign(w, w + g->E);
+  return upload_rows(g);
 }
 
 int gst_groups_set(gst_graph* g, const int64_t* off, const int32_t* mem, int32_t G) {

@@ -80,6 +80,19 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
   return n ? n : 1u;
 }
 
+// Rows are sorted by weight: first position in [lo, hi) whose weight exceeds
+// `limit` (edges past it cannot lead to a useful relaxation).
+__device__ inline int32_t row_cut(const int32_t* wt, int32_t lo, int32_t hi, int32_t limit) {
+  if (limit >= INF) return hi;
+  if (limit < 1) return lo;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (wt[mid] <= limit) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Lower bound of any tree that contains u and touches every group in U
 // (U != 0), from the one-label distances cs = c_i(u): the largest distance,
 // and the two tour bounds of PrunedDP++ (code/PrunedDP++/include/
@@ -87,9 +100,11 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
 // group sequence i ... j visiting all of U, R2[i] = min_j R1[i][j].
 // Also returns sum_{i in U} c_i(u) for the one-label upper bound (INF when
 // some distance is a clamped value >= radius). Returns INF when some group of
-// U is unreachable from u.
+// U is unreachable from u; returns early (without the tour terms) once the
+// largest distance alone exceeds `limit`. Since every tour bound is at most
+// the distance sum, a state that yields an upper bound never returns early.
 __device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_t U,
-                                  int32_t radius, int32_t* sumc) {
+                                  int32_t radius, int32_t limit, int32_t* sumc) {
   const int4 a = *reinterpret_cast<const int4*>(crow);
   const int4 b = *reinterpret_cast<const int4*>(crow + 4);
   const int32_t cs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
@@ -102,9 +117,9 @@ __device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_
       mn = min(mn, cs[i]);
       sum += cs[i];
     }
-  if (one >= INF) {
+  if (one >= INF || one > limit) {  // unreachable, or already past the caller's margin
     *sumc = INF;
-    return INF;
+    return one;
   }
   // distances >= radius are clamped lower bounds, not path lengths
   *sumc = (sum >= INF || one >= radius) ? INF : (int32_t)sum;
@@ -270,15 +285,17 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       int4 a = *reinterpret_cast<const int4*>(row);
       int4 b = *reinterpret_cast<const int4*>(row + 4);
       int32_t cv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-      bool live = false;
+      int32_t lo = INF;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
         if (i >= S.k || cv[i] >= radius) cv[i] = INF;
-        live = live || cv[i] < INF;
+        lo = min(lo, cv[i]);
       }
-      if (live) {
+      if (lo < INF) {
+        // a relaxation must stay below the radius: weight <= radius - 1 - lo
         const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
-        const int32_t re = min(c.rowp[v + 1], rb + CH);
+        const int32_t re = row_cut(c.wt, rb, min(c.rowp[v + 1], rb + CH),
+                                   radius < INF ? radius - 1 - lo : INF);
         deg = max(0, re - rb);
         s_beg[tid] = rb;
         s_cv[tid][0] = make_int4(cv[0], cv[1], cv[2], cv[3]);
@@ -540,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
               int32_t sumc;
               const uint32_t U = (uint32_t)S.full ^ p;
               (void)lb_eval(&c.C[S.c_off + (int64_t)v * CROW], &c.R[S.r_off + U * RSTRIDE], U,
-                            c.sched[tk.slot].R, &sumc);
+                            c.sched[tk.slot].R, INF, &sumc);
               if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
             }
             if (atomicMax(&c.E[S.e_off + (int64_t)p * c.Vp + v], stamp) < stamp) {
@@ -647,8 +664,16 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       if (expand) {
         if (!sc->active) sc->active = 1;
         nF += ch == 0;
+        // a grown state costlier than best + 2 (hop: best) can neither be
+        // queued nor lie on a solution better than best: skip those edges
+        const int32_t bnow = best_of(&c.bestw[s]);
+        int32_t limit = INF;
+        if (bnow < INF) {
+          if (!HOP) limit = bnow + 2 - t;
+          else if (c.hop_prune) limit = bnow - t;
+        }
         const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
-        const int32_t re = min(c.rowp[v + 1], rb + CH);
+        const int32_t re = row_cut(c.wt, rb, min(c.rowp[v + 1], rb + CH), limit);
         deg = (HOP && (int32_t)d >= Dq) ? 0 : max(0, re - rb);
         s_beg[tid] = rb;
         s_cost[tid] = t;
@@ -662,6 +687,25 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
 
     // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
     {
+      // A merged tree costlier than best + 2 (hop: best) is useless, and
+      // T(v, p2) >= max_{i in p2} c_i(v), so partners whose one-label bound
+      // already exceeds the margin are skipped without reading T.
+      int32_t mlim = INF;
+      int32_t cvm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      if (mask) {
+        const int32_t bnow = best_of(&c.bestw[s]);
+        if (bnow < INF) {
+          if (!HOP) mlim = bnow + 2 - t;
+          else if (c.hop_prune) mlim = bnow - t;
+        }
+        if (mlim < INF) {
+          const int32_t* crow = &c.C[S.c_off + (int64_t)v * CROW];
+          const int4 a = *reinterpret_cast<const int4*>(crow);
+          const int4 b = *reinterpret_cast<const int4*>(crow + 4);
+          cvm[0] = a.x; cvm[1] = a.y; cvm[2] = a.z; cvm[3] = a.w;
+          cvm[4] = b.x; cvm[5] = b.y; cvm[6] = b.z; cvm[7] = b.w;
+        }
+      }
       uint32_t p2, d2 = 0;
       if (HOP) p2 = mask;                 // every non-empty submask, complement included
       else p2 = (mask - 1u) & mask;       // proper submasks; the complement was done above
@@ -670,40 +714,53 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         uint32_t want = 0;
         uint64_t item = 0;
         if (live) {
-          ++nRm;
+          int32_t lbp = 0;
+#pragma unroll
+          for (int i = 0; i < 8; ++i)
+            if ((p2 >> i) & 1u) lbp = max(lbp, cvm[i]);
           const int64_t src = HOP ? ((int64_t)d2 * S.nsub + p2) : (int64_t)p2;
-          const int32_t t2 = c.T[S.t_off + src * Vp + v];
-          if (t2 < INF) {
+          const int32_t t2 = lbp > mlim ? INF : c.T[S.t_off + src * Vp + v];
+          nRm += lbp <= mlim;
+          if (t2 < INF && t2 <= mlim) {
             const int32_t mc = t + t2;
             const uint32_t P = p | p2;
             const uint32_t nd = HOP ? max(d, d2) : 0u;
             const int64_t drow = HOP ? ((int64_t)nd * S.nsub + P) : (int64_t)P;
             int32_t* dst = &c.T[S.t_off + drow * Vp + v];
-            if (mc < *dst && mc < atomicMin(dst, mc)) {
-              ++nU;
+            if (mc < *dst) {
               if ((int32_t)P == S.full) {
-                ub_update(&c.bestw[s], mc, wit_full(v, nd));
+                if (mc < atomicMin(dst, mc)) {
+                  ++nU;
+                  ub_update(&c.bestw[s], mc, wit_full(v, nd));
+                }
               } else {
+                // Lower bound first: a state that cannot beat best + 2 (hop:
+                // best) lies on no better solution, so its table entry is left
+                // alone and no atomic is issued.
                 const uint32_t U = (uint32_t)S.full ^ P;
-                bool ok = true;
+                bool keep = true;
+                int32_t sumc = INF, best = INF;
                 if (!HOP || c.hop_prune) {
-                  int32_t sumc;
+                  best = best_of(&c.bestw[s]);
+                  const int32_t lim = best >= INF ? INF : (HOP ? best - mc : best + 2 - mc);
                   const int32_t lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW],
-                                             &c.R[S.r_off + U * RSTRIDE], U, c.sched[s].R, &sumc);
+                                             &c.R[S.r_off + U * RSTRIDE], U, c.sched[s].R, lim,
+                                             &sumc);
                   nLb += __popc(U);
-                  int32_t best = best_of(&c.bestw[s]);
+                  keep = lb < INF && lb <= lim;
+                }
+                if (keep && mc < atomicMin(dst, mc)) {
+                  ++nU;
                   if (!HOP && sumc < INF && mc + sumc < best) {
                     ub_update(&c.bestw[s], mc + sumc, wit_onelabel(v, P));
                     best = mc + sumc;
                   }
-                  if (HOP) ok = lb < INF && mc + lb <= best;
-                  else
-                    ok = lb < INF && (int64_t)1000 * mc <= (int64_t)667 * best + 2000 &&
-                         mc + lb <= best + 2;
-                }
-                if (ok && atomicMax(&c.E[S.e_off + drow * Vp + v], next) < next) {
-                  want = nchunks(c, v);
-                  item = mk_item(v, P, nd, 0, s);
+                  // 2/3 rule (Conditional Tree Merging) for the queue only
+                  const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
+                  if (enq && atomicMax(&c.E[S.e_off + drow * Vp + v], next) < next) {
+                    want = nchunks(c, v);
+                    item = mk_item(v, P, nd, 0, s);
+                  }
                 }
               }
             }
@@ -737,29 +794,30 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         const int64_t dbase = s_dst[j];
         int32_t* dst = &c.T[dbase + u];
         ++nRg;
-        if (gc < *dst && gc < atomicMin(dst, gc)) {
-          ++nU;
+        if (gc < *dst) {
           const uint32_t meta = s_meta[j];
           const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
           const SlotDesc Sj = c.slots[sl];
           const uint32_t U = (uint32_t)Sj.full ^ pd;
-          bool ok = true;
-          if (!HOP || c.hop_prune) {
-            int32_t sumc;
+          bool keep = true;
+          int32_t sumc = INF, best = INF;
+          if (!HOP || c.hop_prune) {  // lower bound before the atomic, as for merges
+            best = best_of(&c.bestw[sl]);
+            const int32_t lim = best >= INF ? INF : (HOP ? best - gc : best + 2 - gc);
             const int32_t lb = lb_eval(&c.C[Sj.c_off + (int64_t)u * CROW],
-                                       &c.R[Sj.r_off + U * RSTRIDE], U, c.sched[sl].R, &sumc);
+                                       &c.R[Sj.r_off + U * RSTRIDE], U, c.sched[sl].R, lim, &sumc);
             nLb += __popc(U);
-            int32_t best = best_of(&c.bestw[sl]);
-            if (!HOP && sumc < INF && gc + sumc < best) {
-              ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
-              best = gc + sumc;
-            }
-            ok = HOP ? (lb < INF && gc + lb <= best) : (lb < INF && gc + lb <= best + 2);
+            keep = lb < INF && lb <= lim;
           }
-          const int64_t erow = dbase - Sj.t_off + Sj.e_off;
-          if (ok && atomicMax(&c.E[erow + u], next) < next) {
-            want = nchunks(c, u);
-            item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+          if (keep && gc < atomicMin(dst, gc)) {
+            ++nU;
+            if (!HOP && sumc < INF && gc + sumc < best)
+              ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+            const int64_t erow = dbase - Sj.t_off + Sj.e_off;
+            if (atomicMax(&c.E[erow + u], next) < next) {
+              want = nchunks(c, u);
+              item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+            }
           }
         }
       }
