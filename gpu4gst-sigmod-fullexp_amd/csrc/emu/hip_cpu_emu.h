@@ -28,6 +28,9 @@ struct int4 {
   int x, y, z, w;
 };
 inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
+struct alignas(8) int2 {
+  int x, y;
+};
 
 #define __global__
 #define __device__
@@ -222,10 +225,11 @@ inline Graph*& capturing() {
 template <class F, class... A>
 void launch(dim3 grid, dim3 block, F kernel, A... args) {
   auto run = [=]() {
+    for (uint32_t bz = 0; bz < grid.z; ++bz)
     for (uint32_t by = 0; by < grid.y; ++by)
       for (uint32_t bx = 0; bx < grid.x; ++bx) {
         Block b;
-        b.bid = dim3(bx, by, 0);
+        b.bid = dim3(bx, by, bz);
         b.bdim = block;
         b.gdim = grid;
         b.entry = [=]() { kernel(args...); };

@@ -146,7 +146,7 @@ int32_t rows_of(int32_t k, int32_t D) { return (D < 0 ? 1 : D + 1) << k; }
 // Table footprint of one slot (int32 elements) and its worklist share.
 int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
   const int64_t rows = rows_of(k, D);
-  return rows * g->Vp + std::max<int64_t>(rows, 8) * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
+  return 2 * rows * g->Vp + std::max<int64_t>(rows, 8) * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
 }
 
 int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
@@ -299,7 +299,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   int64_t cap = std::max((int64_t)((double)items * wl_frac), sssp_items) + 4096;
   if (cap > (int64_t)UINT32_MAX - 65536) cap = (int64_t)UINT32_MAX - 65536;
   bool grew = false;
-  HIP_TRY(g->T.ensure((size_t)t_tot * 4, &grew));
+  HIP_TRY(g->T.ensure((size_t)t_tot * 8, &grew));
   bool e_grew = false;
   HIP_TRY(g->Es.ensure((size_t)e_tot * 4, &e_grew));
   if (e_grew) HIP_TRY(hipMemsetAsync(g->Es.p, 0, g->Es.bytes, g->stream));
@@ -351,7 +351,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   }
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
   launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
-  launch_init_slot_tables(c, nslots, t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
+  launch_init_slot_tables(c, nslots, 2 * t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
   const Task* dt = g->tasks.as<Task>();
   const int nt = (int)tasks.size();
   uint32_t* cnt = g->cnt.as<uint32_t>();
@@ -365,6 +365,9 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   launch_clamp(c, nslots, g->stream);
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
   launch_route(c, nslots, g->stream);
+  int max_nsub = 0;
+  for (const SlotDesc& d : sd) max_nsub = std::max(max_nsub, d.nsub);
+  launch_lb_table(c, nslots, max_nsub, g->stream);
   launch_dp_seed(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(),
                  cnt, g->stream);
   launch_dp_sched_init(c, nslots, g->stream);
@@ -762,8 +765,8 @@ int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_gro
   if (out_edge_offsets) out_edge_offsets[nq] = pos;
   local.seconds_total =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  local.bytes_alg = 16 * local.states_expanded + 12 * local.grow_relax + 8 * local.merge_relax +
-                    12 * local.improvements + 4 * local.lb_terms;
+  local.bytes_alg = 24 * local.states_expanded + 16 * local.grow_relax + 12 * local.merge_relax +
+                    16 * local.improvements + 32 * local.lb_terms;
   if (st) *st = local;
   if (over && (out_edge_uv || out_edge_ids))
     return fail(GST_ERR_CAPACITY, "edge_cap too small; offsets hold the needed size");

@@ -51,7 +51,8 @@ __host__ __device__ inline uint32_t wit_onelabel(uint32_t v, uint32_t p) {
 
 // Per-query-slot layout; all offsets count int32 elements of their arena.
 struct SlotDesc {
-  int64_t t_off;   // cost table T[(d * nsub + p) * Vp + v]; row 0 holds the group mask of v
+  int64_t t_off;   // state words: entry (d * nsub + p) * Vp + v holds {cost, completion bound}
+                   // as two int32; the cost word of row 0 holds the group mask of v
   int64_t e_off;   // enqueue stamps, same indexing as T (rows 0..k-1 serve the one-label phase)
   int64_t c_off;   // one-label distances c[v * 8 + i]
   int64_t r_off;   // route tables [p][72]
@@ -144,6 +145,7 @@ void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
                              hipStream_t st);
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
+void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st);
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
 
 }  // namespace gst

@@ -135,6 +135,23 @@ __device__ inline int32_t lb_eval(const int32_t* crow, const int32_t* R, uint32_
   return max(one, max(lb1, lb2));
 }
 
+// sum_{i in U} c_i(u), or INF if some distance is unreachable or a clamped
+// value >= radius (then it is a bound, not a path length).
+__device__ inline int32_t sum_c(const int32_t* crow, uint32_t U, int32_t radius) {
+  const int4 a = *reinterpret_cast<const int4*>(crow);
+  const int4 b = *reinterpret_cast<const int4*>(crow + 4);
+  const int32_t cs[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  int64_t sum = 0;
+  bool ok = true;
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if ((U >> i) & 1u) {
+      ok = ok && cs[i] < radius && cs[i] < INF;
+      sum += cs[i];
+    }
+  return ok && sum < INF ? (int32_t)sum : INF;
+}
+
 // Exclusive prefix sum over the 256 threads of a workgroup; s_off[256] = total.
 __device__ inline void block_scan(int32_t x, int32_t* s_off, int32_t* s_wsum) {
   const int tid = threadIdx.x, lane = lane_id(), wid = tid / WAVE;
@@ -192,7 +209,7 @@ __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t val) {
 __global__ void k_zero_row0(DevCtx c) {
   const SlotDesc S = c.slots[blockIdx.y];
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.Vp; v += gridDim.x * blockDim.x)
-    c.T[S.t_off + v] = 0;
+    c.T[2 * (S.t_off + v)] = 0;
 }
 
 void launch_fill_i32(int32_t* p, int64_t n, int32_t val, hipStream_t st) {
@@ -236,7 +253,7 @@ __global__ __launch_bounds__(BLOCK) void k_groups_setup(DevCtx c, const Task* ta
       uint32_t v = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        atomicOr(&c.T[S.t_off + v], 1 << tk.bit);
+        atomicOr(&c.T[2 * (S.t_off + v)], 1 << tk.bit);
         atomicMin(&c.C[S.c_off + (int64_t)v * CROW + tk.bit], 0);
         if (atomicMax(&c.E[S.e_off + v], stamp) < stamp) want = nchunks(c, v);
       }
@@ -523,6 +540,40 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
+// Completion bounds: the second word of every state (v, p[, d]) holds
+// lb(v, full ^ p), the lower bound of any tree containing v that touches the
+// groups outside p (the reference's lb0 table, DPQ.cu:193-243, evaluated here
+// with exact group-route tables). A relaxation then decides from one 8-byte
+// read whether its target can still beat the upper bound.
+__global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
+  __shared__ int32_t s_r[RSTRIDE];
+  const int s = blockIdx.z;
+  const SlotDesc S = c.slots[s];
+  const uint32_t p = blockIdx.y + 1;
+  if ((int32_t)p >= S.nsub) return;
+  const uint32_t U = (uint32_t)S.full ^ p;
+  for (int x = threadIdx.x; x < RSTRIDE; x += blockDim.x) s_r[x] = c.R[S.r_off + U * RSTRIDE + x];
+  __syncthreads();
+  const int layers = S.D < 0 ? 1 : S.D + 1;
+  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.V; v += gridDim.x * blockDim.x) {
+    int32_t lb = 0;
+    if (U) {
+      int32_t sumc;
+      lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW], s_r, U, INF, INF, &sumc);
+    }
+    for (int d = 0; d < layers; ++d)
+      c.T[2 * (S.t_off + ((int64_t)d * S.nsub + p) * c.Vp + v) + 1] = lb;
+  }
+}
+
+void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) {
+  if (nslots <= 0 || max_nsub <= 1) return;
+  const int bx = (c.V + 255) / 256 < 64 ? (c.V + 255) / 256 : 64;
+  dim3 grid(bx, max_nsub - 1, nslots);
+  hipLaunchKernelGGL(k_lb_table, grid, 256, 0, st, c);
+}
+
+// ---------------------------------------------------------------------------
 // DP seeds: T(v, p[, 0]) = 0 for every non-empty p inside v's query groups.
 // Each vertex is seeded from its lowest query group only.
 
@@ -540,7 +591,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
       uint32_t v = 0, gsv = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        gsv = (uint32_t)c.T[S.t_off + v];
+        gsv = (uint32_t)c.T[2 * (S.t_off + v)];
         if ((uint32_t)__builtin_ctz(gsv | 0x100u) != (uint32_t)tk.bit) gsv = 0;
       }
       uint32_t p = gsv;  // enumerate non-empty submasks of gsv
@@ -549,15 +600,13 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
         uint32_t want = 0;
         uint64_t item = 0;
         if (live) {
-          atomicMin(&c.T[S.t_off + (int64_t)p * c.Vp + v], 0);
+          atomicMin(&c.T[2 * (S.t_off + (int64_t)p * c.Vp + v)], 0);
           if ((int32_t)p == S.full) {
             ub_update(&c.bestw[tk.slot], 0, wit_full(v, 0));
           } else {
             if (!hop) {
-              int32_t sumc;
-              const uint32_t U = (uint32_t)S.full ^ p;
-              (void)lb_eval(&c.C[S.c_off + (int64_t)v * CROW], &c.R[S.r_off + U * RSTRIDE], U,
-                            c.sched[tk.slot].R, INF, &sumc);
+              const int32_t sumc = sum_c(&c.C[S.c_off + (int64_t)v * CROW], (uint32_t)S.full ^ p,
+                                         c.sched[tk.slot].R);
               if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
             }
             if (atomicMax(&c.E[S.e_off + (int64_t)p * c.Vp + v], stamp) < stamp) {
@@ -635,18 +684,22 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       S = c.slots[s];
       Dq = S.D;
       const int64_t row = HOP ? (int64_t)d * S.nsub + p : (int64_t)p;
-      t = c.T[S.t_off + row * Vp + v];
+      const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * (S.t_off + row * Vp + v)]);
+      t = tl.x;
       Sched* sc = &c.sched[s];
       bool expand = t < INF;
-      if (!HOP) {
-        const int32_t best = best_of(&c.bestw[s]);
+      const int32_t best = best_of(&c.bestw[s]);
+      // the state's own completion bound: useless once t + lb exceeds best + 2 (hop: best)
+      if (!HOP && t + tl.y > best + 2) expand = false;
+      else if (HOP && c.hop_prune && best < INF && t + tl.y > best) expand = false;
+      else if (!HOP) {
         const bool half = t < best / 2 + 2;  // 1/2 rule: costlier states are never expanded
         if (ch == 0 && (!half || t < sc->thr)) {
           const uint32_t xs = (uint32_t)S.full ^ p;
-          const int32_t t2 = c.T[S.t_off + (int64_t)xs * Vp + v];
+          const int32_t t2 = c.T[2 * (S.t_off + (int64_t)xs * Vp + v)];
           if (t2 < INF && t < INF) {
             const int32_t mcost = t + t2;
-            int32_t* dfull = &c.T[S.t_off + (int64_t)S.full * Vp + v];
+            int32_t* dfull = &c.T[2 * (S.t_off + (int64_t)S.full * Vp + v)];
             if (mcost < *dfull && mcost < atomicMin(dfull, mcost))
               ub_update(&c.bestw[s], mcost, wit_full(v, 0));
           }
@@ -719,41 +772,41 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           for (int i = 0; i < 8; ++i)
             if ((p2 >> i) & 1u) lbp = max(lbp, cvm[i]);
           const int64_t src = HOP ? ((int64_t)d2 * S.nsub + p2) : (int64_t)p2;
-          const int32_t t2 = lbp > mlim ? INF : c.T[S.t_off + src * Vp + v];
+          const int32_t t2 = lbp > mlim ? INF : c.T[2 * (S.t_off + src * Vp + v)];
           nRm += lbp <= mlim;
           if (t2 < INF && t2 <= mlim) {
             const int32_t mc = t + t2;
             const uint32_t P = p | p2;
             const uint32_t nd = HOP ? max(d, d2) : 0u;
             const int64_t drow = HOP ? ((int64_t)nd * S.nsub + P) : (int64_t)P;
-            int32_t* dst = &c.T[S.t_off + drow * Vp + v];
-            if (mc < *dst) {
+            int32_t* dst = &c.T[2 * (S.t_off + drow * Vp + v)];
+            const int2 dl = *reinterpret_cast<const int2*>(dst);
+            if (mc < dl.x) {
               if ((int32_t)P == S.full) {
                 if (mc < atomicMin(dst, mc)) {
                   ++nU;
                   ub_update(&c.bestw[s], mc, wit_full(v, nd));
                 }
               } else {
-                // Lower bound first: a state that cannot beat best + 2 (hop:
-                // best) lies on no better solution, so its table entry is left
-                // alone and no atomic is issued.
-                const uint32_t U = (uint32_t)S.full ^ P;
+                // Completion bound first: a state that cannot beat best + 2
+                // (hop: best) lies on no better solution, so its entry is
+                // left alone and no atomic is issued.
+                int32_t best = INF;
                 bool keep = true;
-                int32_t sumc = INF, best = INF;
                 if (!HOP || c.hop_prune) {
                   best = best_of(&c.bestw[s]);
-                  const int32_t lim = best >= INF ? INF : (HOP ? best - mc : best + 2 - mc);
-                  const int32_t lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW],
-                                             &c.R[S.r_off + U * RSTRIDE], U, c.sched[s].R, lim,
-                                             &sumc);
-                  nLb += __popc(U);
-                  keep = lb < INF && lb <= lim;
+                  keep = dl.y < INF && (best >= INF || mc + dl.y <= (HOP ? best : best + 2));
                 }
                 if (keep && mc < atomicMin(dst, mc)) {
                   ++nU;
-                  if (!HOP && sumc < INF && mc + sumc < best) {
-                    ub_update(&c.bestw[s], mc + sumc, wit_onelabel(v, P));
-                    best = mc + sumc;
+                  if (!HOP) {
+                    ++nLb;
+                    const int32_t sumc = sum_c(&c.C[S.c_off + (int64_t)v * CROW],
+                                               (uint32_t)S.full ^ P, c.sched[s].R);
+                    if (sumc < INF && mc + sumc < best) {
+                      ub_update(&c.bestw[s], mc + sumc, wit_onelabel(v, P));
+                      best = mc + sumc;
+                    }
                   }
                   // 2/3 rule (Conditional Tree Merging) for the queue only
                   const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
@@ -792,27 +845,28 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         const uint32_t u = (uint32_t)c.col[ej];
         const int32_t gc = s_cost[j] + c.wt[ej];
         const int64_t dbase = s_dst[j];
-        int32_t* dst = &c.T[dbase + u];
+        int32_t* dst = &c.T[2 * (dbase + u)];
+        const int2 dl = *reinterpret_cast<const int2*>(dst);
         ++nRg;
-        if (gc < *dst) {
+        if (gc < dl.x) {
           const uint32_t meta = s_meta[j];
           const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
-          const SlotDesc Sj = c.slots[sl];
-          const uint32_t U = (uint32_t)Sj.full ^ pd;
+          int32_t best = INF;
           bool keep = true;
-          int32_t sumc = INF, best = INF;
-          if (!HOP || c.hop_prune) {  // lower bound before the atomic, as for merges
+          if (!HOP || c.hop_prune) {  // completion bound before the atomic, as for merges
             best = best_of(&c.bestw[sl]);
-            const int32_t lim = best >= INF ? INF : (HOP ? best - gc : best + 2 - gc);
-            const int32_t lb = lb_eval(&c.C[Sj.c_off + (int64_t)u * CROW],
-                                       &c.R[Sj.r_off + U * RSTRIDE], U, c.sched[sl].R, lim, &sumc);
-            nLb += __popc(U);
-            keep = lb < INF && lb <= lim;
+            keep = dl.y < INF && (best >= INF || gc + dl.y <= (HOP ? best : best + 2));
           }
           if (keep && gc < atomicMin(dst, gc)) {
             ++nU;
-            if (!HOP && sumc < INF && gc + sumc < best)
-              ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+            const SlotDesc Sj = c.slots[sl];
+            if (!HOP) {
+              ++nLb;
+              const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)u * CROW], (uint32_t)Sj.full ^ pd,
+                                         c.sched[sl].R);
+              if (sumc < INF && gc + sumc < best)
+                ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+            }
             const int64_t erow = dbase - Sj.t_off + Sj.e_off;
             if (atomicMax(&c.E[erow + u], next) < next) {
               want = nchunks(c, u);
@@ -892,7 +946,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     return;
   }
   auto Tat = [&](uint32_t v, uint32_t p, uint32_t d) -> int32_t {
-    return c.T[S.t_off + ((int64_t)d * S.nsub + p) * Vp + v];
+    return c.T[2 * (S.t_off + ((int64_t)d * S.nsub + p) * Vp + v)];
   };
   auto emit = [&](uint32_t a, uint32_t b2, int32_t ej) {  // lane 0 only
     if (s_ne >= max_edges) {
@@ -968,7 +1022,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     const int32_t b = st_b[top];
     __syncthreads();
     if (lane == 0) s_sp = top;
-    const uint32_t gs = (uint32_t)c.T[S.t_off + v];
+    const uint32_t gs = (uint32_t)c.T[2 * (S.t_off + v)];
     __syncthreads();
     if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
     // merge splits: proper non-empty submasks p1 (hop: with layers d1, d2,

@@ -71,7 +71,7 @@ typedef struct gst_stats {
   int64_t grow_relax;         /* Rg: DP edge relaxations */
   int64_t merge_relax;        /* Rm: DP merge candidates evaluated */
   int64_t improvements;       /* U: successful DP table decreases */
-  int64_t lb_terms;           /* Lb: one-label reads made by lower-bound checks */
+  int64_t lb_terms;           /* Lb: one-label rows read for upper bounds (32 B each) */
   int64_t items_processed;    /* DP worklist items (chunks included) */
   int64_t max_frontier;       /* largest DP worklist */
   int64_t bytes_alg;          /* algorithmic bytes of the DP expand kernel */
