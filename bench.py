@@ -67,8 +67,14 @@ def main():
     ap.add_argument("--sssp-delta", type=int, default=-1)
     ap.add_argument("--dp-div", type=int, default=-1)
     ap.add_argument("--grid", type=int, default=0)
-    ap.add_argument("--single-chunk", type=int, default=0,
-                    help="virtual-split chunk for the single-query path (0 = same as batched)")
+    ap.add_argument("--radius-pct", type=int, default=0)
+    ap.add_argument("--max-batch", type=int, default=0,
+                    help="cap on queries per lockstep batch (0 = as many as the HBM budget admits)")
+    ap.add_argument("--single-chunk", type=int, default=256,
+                    help="virtual-split chunk for the single-query path (its best setting; "
+                         "0 = same as batched)")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a device copy of this size (counter calibration) and exit")
     ap.add_argument("--single-dp-div", type=int, default=-1)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -77,6 +83,17 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_probe_mib:
+        # reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE: reads and
+        # writes exactly copy_probe_mib MiB
+        n = args.copy_probe_mib << 18
+        a = torch.ones(n, dtype=torch.int32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": args.copy_probe_mib}))
+        return
     from gpu4gst import GSTGraph, workloads
     t0 = time.perf_counter()
     w = workloads.config(2, n_queries=args.queries)
@@ -93,6 +110,10 @@ def main():
         G.set_option("dp_delta_div", args.dp_div)
     if args.grid:
         G.set_option("grid", args.grid)
+    if args.radius_pct:
+        G.set_option("radius_pct", args.radius_pct)
+    if args.max_batch:
+        G.set_option("max_batch", args.max_batch)
     torch.cuda.synchronize()
 
     for i in range(args.warmup):
@@ -161,13 +182,17 @@ def main():
         "config": {"workload": w.name, "V": w.graph.n, "E_undirected": w.graph.m,
                    "groups": w.groups.count, "queries_per_step": len(w.queries),
                    "k": [3, 5, 7], "hop_bound": None},
+        "phase_ms": st["phase_ms"],
         "single_query_qps": single_qps,
+        "single_query_chunk": args.single_chunk,
         "batched_over_single": qps / single_qps if single_qps else None,
         "single_sample": ns,
         "cost_mismatch_batched_vs_single": mism_single,
         "costs_stable_across_steps": stable,
         "state_relaxations_per_s": relax * args.steps / el,
-        "dp_stats": {k: st[k] for k in ("sssp_iterations", "dp_iterations", "states_expanded",
+        "dp_stats": {k: st[k] for k in ("sssp_iterations", "sssp_items", "sssp_relax",
+                                         "sssp_updates", "sssp_items_unbounded",
+                                         "dp_iterations", "states_expanded",
                                          "grow_relax", "merge_relax", "improvements",
                                          "max_frontier", "batches")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -31,6 +31,8 @@ inline int4 make_int4(int a, int b, int c, int d) { return int4{a, b, c, d}; }
 struct alignas(8) int2 {
   int x, y;
 };
+inline int2 make_int2(int a, int b) { return int2{a, b}; }
+inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 
 #define __global__
 #define __device__
@@ -314,6 +316,12 @@ template <class T>
 inline T atomicOr(T* p, T v) {
   T o = *p;
   *p = o | v;
+  return o;
+}
+template <class T>
+inline T atomicCAS(T* p, T cmp, T v) {
+  T o = *p;
+  if (o == cmp) *p = v;
   return o;
 }
 template <class T>

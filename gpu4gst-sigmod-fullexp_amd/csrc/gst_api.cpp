@@ -95,7 +95,7 @@ struct gst_graph {
   std::vector<int64_t> h_goff;
   DBuf goff, gmem;
   // options
-  int32_t chunk_shift = 9;
+  int32_t chunk_shift = 12;
   int32_t grid = 0;
   int32_t iters_per_sync = 32;
   int32_t hop_prune = 1;
@@ -103,8 +103,10 @@ struct gst_graph {
   int32_t infeasible_cost = INT32_MAX;
   int32_t max_tree_edges = 4096;
   int32_t use_graphs = 1;
-  int32_t sssp_delta = 0;  // 0 = max(1, largest weight / 8)
-  int32_t dp_div = 16;
+  int32_t sssp_delta = 0;  // 0 = unordered (bands of this width for the hop DP otherwise)
+  int32_t dp_div = 0;
+  int32_t radius_pct = 100;
+  int32_t max_batch = 0;
   int32_t maxw = 1;
   // engine buffers (grow-only)
   DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
@@ -114,6 +116,7 @@ struct gst_graph {
   int64_t gen = 0;  // bumps whenever a pointer or scalar baked into a captured graph changes
   std::map<std::pair<int, int>, std::pair<int64_t, hipGraphExec_t>> graphs;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};  // phase boundaries
   int64_t nch_total() const {
     int64_t s = 0;
     const int32_t CH = 1 << chunk_shift;
@@ -142,6 +145,11 @@ struct gst_graph {
 namespace {
 
 int32_t rows_of(int32_t k, int32_t D) { return (D < 0 ? 1 : D + 1) << k; }
+
+// Worklists are sized to this fraction of all DP states (frontiers observed on
+// power-law graphs stay near 1%); a batch that overflows is re-run with
+// worklists as large as the state count, which can never overflow.
+constexpr double kWorklistFraction = 0.04;
 
 // Table footprint of one slot (int32 elements) and its worklist share.
 int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
@@ -173,8 +181,9 @@ DevCtx make_ctx(gst_graph* g) {
   c.cap = g->wl_cap;
   c.sched = g->sched.as<Sched>();
   c.nslots = g->iter.as<int32_t>() + 1;
-  c.sssp_delta = g->sssp_delta > 0 ? g->sssp_delta : std::max(1, g->maxw / 8);
+  c.sssp_delta = g->sssp_delta > 0 ? g->sssp_delta : INF / 2;
   c.dp_div = g->dp_div;
+  c.radius_pct = g->radius_pct;
   return c;
 }
 
@@ -292,7 +301,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     t_tot += rows * g->Vp;
     e_tot += std::max<int64_t>(rows, 8) * g->Vp;
     items += slot_items(g, q.k, q.D);
-    sssp_items += (int64_t)q.k * g->nch();
+    sssp_items += g->nch();  // one-label items are (slot, vertex[, chunk])
     sd[s] = d;
     for (int i = 0; i < q.k; ++i) tasks.push_back(Task{s, i, q.groups[i], 0});
   }
@@ -349,6 +358,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
                            hipMemcpyHostToDevice, g->stream));
     HIP_TRY(hipStreamSynchronize(g->stream));
   }
+  HIP_TRY(hipEventRecord(g->ph[0], g->stream));
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
   launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
   launch_init_slot_tables(c, nslots, 2 * t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
@@ -362,6 +372,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   int64_t dp_launches = 0, sssp_launches = 0;
   int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
   if (rc) return rc;  // includes 100 = overflow
+  HIP_TRY(hipEventRecord(g->ph[1], g->stream));
   launch_clamp(c, nslots, g->stream);
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
   launch_route(c, nslots, g->stream);
@@ -372,8 +383,10 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
                  cnt, g->stream);
   launch_dp_sched_init(c, nslots, g->stream);
   HIP_TRY(hipGetLastError());
+  HIP_TRY(hipEventRecord(g->ph[2], g->stream));
   rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
   if (rc) return rc;
+  HIP_TRY(hipEventRecord(g->ph[3], g->stream));
   launch_backtrack(c, nslots, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
                    g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), g->stream);
   HIP_TRY(hipGetLastError());
@@ -385,6 +398,8 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(hipMemcpyAsync(oerr.data(), g->out_err.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (ctr[C_OVERFLOW]) return 100;  // worklist too small: caller retries with full capacity
+  float ph_ms[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i) HIP_TRY(hipEventElapsedTime(&ph_ms[i], g->ph[i], g->ph[i + 1]));
   std::vector<int32_t> huv((size_t)nslots * maxe * 2), hid((size_t)nslots * maxe);
   HIP_TRY(hipMemcpy(huv.data(), g->out_uv.p, huv.size() * 4, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(hid.data(), g->out_eid.p, hid.size() * 4, hipMemcpyDeviceToHost));
@@ -424,6 +439,8 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     st->dp_iterations += (int64_t)ctr[C_DP_ITERS];
     st->sssp_items += (int64_t)ctr[C_SSSP_ITEMS];
     st->sssp_relax += (int64_t)ctr[C_SSSP_RELAX];
+    st->sssp_updates += (int64_t)ctr[C_SSSP_UPD];
+    st->sssp_items_unbounded += (int64_t)ctr[C_SSSP_NOUB];
     st->states_expanded += (int64_t)ctr[C_F];
     st->grow_relax += (int64_t)ctr[C_RG];
     st->merge_relax += (int64_t)ctr[C_RM];
@@ -434,6 +451,9 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     st->expand_ms += dp_ms;
     st->expand_launches += dp_launches;
     st->sssp_expand_ms += sssp_ms;
+    st->phase_ms[0] += ph_ms[0];
+    st->phase_ms[1] += ph_ms[1];
+    st->phase_ms[2] += ph_ms[2];
   }
   return GST_OK;
 }
@@ -446,7 +466,7 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
     HIP_TRY(hipMemGetInfo(&fr, &tot));
     budget = (int64_t)((double)fr * 0.8);
   }
-  const double frac = 0.25;
+  const double frac = kWorklistFraction;
   // stable order: plain queries first, then hop queries (one mode per batch)
   std::stable_sort(qs.begin(), qs.end(),
                    [](const QueryIn& a, const QueryIn& b) { return (a.D >= 0) < (b.D >= 0); });
@@ -455,7 +475,8 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
     std::vector<QueryIn> batch;
     int64_t used = 0;
     const bool hop = qs[i].D >= 0;
-    while (i < qs.size() && (qs[i].D >= 0) == hop && batch.size() < MAX_SLOTS) {
+    const size_t cap_slots = g->max_batch > 0 ? std::min<size_t>(g->max_batch, MAX_SLOTS) : MAX_SLOTS;
+    while (i < qs.size() && (qs[i].D >= 0) == hop && batch.size() < cap_slots) {
       const int64_t f = gst_query_footprint(g, qs[i].k, qs[i].D);
       if (!batch.empty() && used + f > budget) break;
       if (batch.empty() && f > budget)
@@ -589,6 +610,8 @@ int gst_graph_create(const void* beg_pos, const void* csr, const void* weight, i
   }
   if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
+      hipEventCreate(&g->ph[0]) != hipSuccess || hipEventCreate(&g->ph[1]) != hipSuccess ||
+      hipEventCreate(&g->ph[2]) != hipSuccess || hipEventCreate(&g->ph[3]) != hipSuccess ||
       hipHostMalloc((void**)&g->h_cnt, 16, 0) != hipSuccess) {
     gst_graph_destroy(g);
     return fail(GST_ERR_DEVICE, "stream / event / pinned allocation failed");
@@ -690,6 +713,14 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
       if (v < 0 || v > INF / 2) return fail(GST_ERR_INVALID, "band width out of range");
       g->sssp_delta = (int32_t)v;
       break;
+    case GST_OPT_MAX_BATCH:
+      if (v < 0 || v > (int64_t)MAX_SLOTS) return fail(GST_ERR_INVALID, "max batch out of range");
+      g->max_batch = (int32_t)v;
+      break;
+    case GST_OPT_RADIUS_PCT:
+      if (v < 1 || v > 100000) return fail(GST_ERR_INVALID, "radius percentage out of range");
+      g->radius_pct = (int32_t)v;
+      break;
     case GST_OPT_DP_DELTA_DIV:
       if (v < 0 || v > (1 << 20)) return fail(GST_ERR_INVALID, "band divisor out of range");
       g->dp_div = (int32_t)v;
@@ -704,7 +735,8 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
 
 int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
   if (!g || k < 1 || k > GST_MAX_K) return -1;
-  const double wl_items = std::max(0.25 * (double)slot_items(g, k, D), (double)k * (double)g->nch());
+  const double wl_items =
+      std::max(kWorklistFraction * (double)slot_items(g, k, D), (double)g->nch());
   return slot_ints(g, k, D) * 4 + (int64_t)(2 * 8 * wl_items) +
          (int64_t)g->max_tree_edges * 12 + 1024;
 }
@@ -809,6 +841,8 @@ void gst_graph_destroy(gst_graph* g) {
   if (g->h_cnt) (void)hipHostFree(g->h_cnt);
   if (g->ev0) (void)hipEventDestroy(g->ev0);
   if (g->ev1) (void)hipEventDestroy(g->ev1);
+  for (hipEvent_t e : g->ph)
+    if (e) (void)hipEventDestroy(e);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
 }

@@ -75,6 +75,8 @@ enum Ctr {
   C_DP_ITERS,
   C_SSSP_ITERS,
   C_OVERFLOW,
+  C_SSSP_UPD,
+  C_SSSP_NOUB,
   C_NUM
 };
 
@@ -111,6 +113,7 @@ struct DevCtx {
   const int32_t* nslots;         // number of slots of the current batch (device)
   int32_t sssp_delta;            // band width of the one-label phase
   int32_t dp_div;                // DP band width = max(1, best / dp_div); 0 = unordered
+  int32_t radius_pct;            // one-label radius = best * radius_pct / 100 + 2
 };
 
 struct WL {

@@ -80,6 +80,15 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
   return n ? n : 1u;
 }
 
+// One-label radius of a plain query: distances at or beyond it are never
+// propagated and are clamped to it afterwards (a valid lower bound). Any
+// radius keeps every bound valid; radius_pct only trades one-label work
+// against lower-bound strength.
+__device__ inline int32_t sssp_radius(const DevCtx& c, int32_t best) {
+  if (best >= INF - 2) return INF;
+  return (int32_t)(((int64_t)best * c.radius_pct) / 100) + 2;
+}
+
 // Rows are sorted by weight: first position in [lo, hi) whose weight exceeds
 // `limit` (edges past it cannot lead to a useful relaxation).
 __device__ inline int32_t row_cut(const int32_t* wt, int32_t lo, int32_t hi, int32_t limit) {
@@ -288,7 +297,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
-  unsigned long long n_items = 0, n_relax = 0;
+  unsigned long long n_items = 0, n_relax = 0, n_upd = 0, n_noub = 0;
   for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
     const uint32_t idx = tile * BLOCK + tid;
     int32_t deg = 0;
@@ -297,7 +306,8 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       const uint32_t v = it_v(it), ch = it_ch(it), s = it_s(it);
       const SlotDesc S = c.slots[s];
       const int32_t best = S.D < 0 ? best_of(&c.bestw[s]) : INF;
-      const int32_t radius = best < INF - 2 ? best + 2 : INF;
+      const int32_t radius = sssp_radius(c, best);
+      n_noub += S.D < 0 && radius >= INF;
       const int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
       int4 a = *reinterpret_cast<const int4*>(row);
       int4 b = *reinterpret_cast<const int4*>(row + 4);
@@ -348,6 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
           const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
           if (cv[i] < INF && cand < cu[i]) {
             const int32_t old = atomicMin(&row[i], cand);
+            ++n_upd;
             if (cand < old) improved = true;
             cu[i] = min(cand, old);
           }
@@ -377,6 +388,8 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   }
   flush(c.ctr, C_SSSP_ITEMS, n_items);
   flush(c.ctr, C_SSSP_RELAX, n_relax);
+  flush(c.ctr, C_SSSP_UPD, n_upd);
+  flush(c.ctr, C_SSSP_NOUB, n_noub);
 }
 
 void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st) {
@@ -419,7 +432,7 @@ __global__ void k_clamp(DevCtx c) {
   int32_t R = INF;
   if (S.D < 0) {
     const int32_t best = best_of(&c.bestw[s]);
-    R = best < INF - 2 ? best + 2 : INF;
+    R = sssp_radius(c, best);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) c.sched[s].R = R;
   if (R >= INF) return;

@@ -16,19 +16,22 @@ EXPORTS = ("gst_graph_create", "gst_graph_set_edge_ids", "gst_graph_set_weights"
 
 OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_INTERNAL = 0, 1, 2, 3, 4, 5
 OPT = dict(chunk=1, grid=2, iters_per_sync=3, hop_prune=4, profile=5, infeasible_cost=6,
-           max_tree_edges=7, use_graphs=8, sssp_delta=10, dp_delta_div=11)
+           max_tree_edges=7, use_graphs=8, sssp_delta=10, dp_delta_div=11, radius_pct=12, max_batch=14)
 
 
 class GstStats(C.Structure):
     _fields_ = [(n, C.c_int64) for n in (
         "queries", "batches", "sssp_iterations", "dp_iterations", "sssp_items", "sssp_relax",
+        "sssp_updates", "sssp_items_unbounded",
         "states_expanded", "grow_relax", "merge_relax", "improvements", "lb_terms",
         "items_processed", "max_frontier", "bytes_alg")] + [
         ("seconds_total", C.c_double), ("expand_ms", C.c_double), ("expand_launches", C.c_int64),
-        ("sssp_expand_ms", C.c_double)]
+        ("sssp_expand_ms", C.c_double), ("phase_ms", C.c_double * 3)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        d = {n: getattr(self, n) for n, _ in self._fields_}
+        d["phase_ms"] = list(self.phase_ms)
+        return d
 
 
 _LIB = None

@@ -66,7 +66,9 @@ typedef struct gst_stats {
   int64_t sssp_iterations;    /* frontier iterations of the one-label phase */
   int64_t dp_iterations;      /* frontier iterations of the DP phase */
   int64_t sssp_items;         /* one-label states expanded */
-  int64_t sssp_relax;         /* one-label edge relaxations */
+  int64_t sssp_relax;         /* one-label edge visits (each relaxes all k labels) */
+  int64_t sssp_updates;       /* one-label atomic updates */
+  int64_t sssp_items_unbounded; /* one-label items seen before the query had an upper bound */
   int64_t states_expanded;    /* F: DP states whose grow/merge ran */
   int64_t grow_relax;         /* Rg: DP edge relaxations */
   int64_t merge_relax;        /* Rm: DP merge candidates evaluated */
@@ -79,10 +81,12 @@ typedef struct gst_stats {
   double expand_ms;           /* DP expand kernel time (profiling mode only) */
   int64_t expand_launches;    /* DP expand launches (profiling mode only) */
   double sssp_expand_ms;      /* one-label expand kernel time (profiling mode only) */
+  double phase_ms[3];         /* device time: one-label phase, bounds + seeds, DP phase */
 } gst_stats;
 
 /* options for gst_set_option */
-#define GST_OPT_CHUNK 1            /* edges per virtual-split chunk (power of two, 64..65536) */
+#define GST_OPT_CHUNK 1            /* edges per virtual-split chunk (power of two, 64..65536;
+                                      default 4096) */
 #define GST_OPT_GRID 2             /* workgroups of the expand kernels (0 = auto) */
 #define GST_OPT_ITERS_PER_SYNC 3   /* max graph-captured iterations between host checks */
 #define GST_OPT_HOP_PRUNE 4        /* 1 = lower-bound pruning in the hop DP (default 1) */
@@ -91,9 +95,11 @@ typedef struct gst_stats {
                                       100000 reproduces the reference's inf) */
 #define GST_OPT_MAX_TREE_EDGES 7   /* per-query edge capacity of the backtracking kernel */
 #define GST_OPT_USE_GRAPHS 8       /* 1 = capture iteration steps in HIP graphs (default 1) */
-#define GST_OPT_SSSP_DELTA 10      /* cost band of the one-label phase (0 = largest weight / 8;
-                                      a huge value = unordered label correcting) */
-#define GST_OPT_DP_DELTA_DIV 11    /* DP cost band = upper bound / value (default 16; 0 = unordered) */
+#define GST_OPT_SSSP_DELTA 10      /* reserved (the one-label phase is unordered) */
+#define GST_OPT_RADIUS_PCT 12      /* one-label radius = upper bound * value / 100 + 2 (default 100);
+                                      distances beyond it are clamped lower bounds */
+#define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */
+#define GST_OPT_DP_DELTA_DIV 11    /* DP cost band = upper bound / value (default 0 = unordered) */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
  * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge
