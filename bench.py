@@ -32,6 +32,27 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def _traffic(workload: str) -> dict:
+    """HBM bytes per launch of each expand kernel on this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE and WRITE_SIZE runs of this
+    benchmark, each scaled by the reading of a copy kernel of known size)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f).get("kernels", {})
+
+
+def _roof(kernel, bytes_alg, kernel_ms, launches, traffic):
+    achieved = bytes_alg / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic["bytes_per_launch"] if traffic else None,
+            "kernel": kernel, "kernel_ms_total": kernel_ms, "launches": launches,
+            "bytes_alg_per_launch": bytes_alg / launches,
+            "kernel_ms_per_launch": kernel_ms / launches}
+
+
 def cpu_baseline(w, budget_s: float, single_costs: dict):
     """Restated PrunedDP++ (oracle, one thread) over the first queries until
     budget_s seconds of CPU work; costs are checked against the GPU's."""
@@ -73,6 +94,8 @@ def main():
     ap.add_argument("--single-chunk", type=int, default=256,
                     help="virtual-split chunk for the single-query path (its best setting; "
                          "0 = same as batched)")
+    ap.add_argument("--profile-only", action="store_true",
+                    help="only run the instrumented solve (the pass rocprofv3 summaries cover)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration) and exit")
     ap.add_argument("--single-dp-div", type=int, default=-1)
@@ -116,6 +139,15 @@ def main():
         G.set_option("max_batch", args.max_batch)
     torch.cuda.synchronize()
 
+    if args.profile_only:
+        G.set_option("profile", 1)
+        pr = G.solve_batch(w.queries)
+        ps = pr.stats
+        print(json.dumps({"profile_only": w.name, "expand_ms": ps["expand_ms"],
+                          "expand_launches": ps["expand_launches"],
+                          "sssp_expand_ms": ps["sssp_expand_ms"],
+                          "sssp_expand_launches": ps["sssp_expand_launches"]}))
+        return
     for i in range(args.warmup):
         r = G.solve_batch(w.queries)
         log(f"warmup {i}: {r.stats['seconds_total']:.3f} s")
@@ -155,16 +187,25 @@ def main():
         G.set_option("chunk", args.chunk or 4096)
     if args.single_dp_div >= 0:
         G.set_option("dp_delta_div", args.dp_div if args.dp_div >= 0 else 0)
-    # instrumented pass: DP expand kernel time with HIP events on the solver's stream
+    # instrumented pass: every expand launch timed with HIP events on the solver's stream
     G.set_option("profile", 1)
     pr = G.solve_batch(w.queries)
     G.set_option("profile", 0)
     ps = pr.stats
-    ek_s = ps["expand_ms"] / 1e3
-    achieved = ps["bytes_alg"] / ek_s / 1e9 if ek_s > 0 else 0.0
-    log(f"profiled: expand {ps['expand_ms']:.1f} ms over {ps['expand_launches']} launches, "
-        f"bytes_alg {ps['bytes_alg']}")
+    log(f"profiled: dp expand {ps['expand_ms']:.1f} ms / {ps['expand_launches']} launches, "
+        f"one-label expand {ps['sssp_expand_ms']:.1f} ms")
     relax = st["grow_relax"] + st["merge_relax"]
+    traffic = _traffic(w.name)
+    # one-label expand: item 8 + row 32 + row bounds 8 B; edge visit: col 4 + weight 4 +
+    # neighbour row 32 B; label update: 4 B
+    sssp_bytes = 48 * ps["sssp_items"] + 40 * ps["sssp_relax"] + 4 * ps["sssp_updates"]
+    sssp_launches = max(1, ps["sssp_expand_launches"])
+    roof_sssp = _roof("k_sssp_expand", sssp_bytes, ps["sssp_expand_ms"], sssp_launches,
+                      traffic.get("k_sssp_expand"))
+    roof_dp = _roof("k_dp_expand<false>", ps["bytes_alg"], ps["expand_ms"],
+                    max(1, ps["expand_launches"]), traffic.get("k_dp_expand<false>"))
+    dominant, other = (roof_sssp, roof_dp) if ps["sssp_expand_ms"] >= ps["expand_ms"] else (
+        roof_dp, roof_sssp)
 
     out = {
         "metric": "GST queries/s at k=3/5/7 (batched, exact cost)",
@@ -195,11 +236,8 @@ def main():
                                          "dp_iterations", "states_expanded",
                                          "grow_relax", "merge_relax", "improvements",
                                          "max_frontier", "batches")},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_dp_expand<false>",
-                     "kernel_ms_total": ps["expand_ms"], "launches": ps["expand_launches"],
-                     "bytes_alg_total": ps["bytes_alg"]},
+        "roofline": dominant,
+        "roofline_other_kernel": other,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, single)

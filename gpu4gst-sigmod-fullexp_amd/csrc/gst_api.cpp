@@ -450,6 +450,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     st->max_frontier = std::max<int64_t>(st->max_frontier, (int64_t)ctr[C_MAX_FRONTIER]);
     st->expand_ms += dp_ms;
     st->expand_launches += dp_launches;
+    st->sssp_expand_launches += sssp_launches;
     st->sssp_expand_ms += sssp_ms;
     st->phase_ms[0] += ph_ms[0];
     st->phase_ms[1] += ph_ms[1];

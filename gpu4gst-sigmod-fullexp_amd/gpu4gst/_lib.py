@@ -26,7 +26,8 @@ class GstStats(C.Structure):
         "states_expanded", "grow_relax", "merge_relax", "improvements", "lb_terms",
         "items_processed", "max_frontier", "bytes_alg")] + [
         ("seconds_total", C.c_double), ("expand_ms", C.c_double), ("expand_launches", C.c_int64),
-        ("sssp_expand_ms", C.c_double), ("phase_ms", C.c_double * 3)]
+        ("sssp_expand_ms", C.c_double), ("phase_ms", C.c_double * 3),
+        ("sssp_expand_launches", C.c_int64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}

@@ -82,6 +82,7 @@ typedef struct gst_stats {
   int64_t expand_launches;    /* DP expand launches (profiling mode only) */
   double sssp_expand_ms;      /* one-label expand kernel time (profiling mode only) */
   double phase_ms[3];         /* device time: one-label phase, bounds + seeds, DP phase */
+  int64_t sssp_expand_launches; /* one-label expand launches (profiling mode only) */
 } gst_stats;
 
 /* options for gst_set_option */
