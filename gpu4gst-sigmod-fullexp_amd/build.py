@@ -17,6 +17,7 @@ OUT = os.path.join(HERE, "gpu4gst")
 BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("GST_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+EMU_LIB = os.path.join(BUILD, "libgst_emu.so")
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -43,6 +44,13 @@ def build(verbose: bool = False) -> None:
     tmp = os.path.join(OUT, "libgst.so.tmp")
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp], verbose)
     os.replace(tmp, os.path.join(OUT, "libgst.so"))
+    # the same kernel source built for the CPU (csrc/emu/hip_cpu_emu.h): used by the
+    # CPU test suite to check kernel logic against the oracle without a GPU
+    tmp = os.path.join(BUILD, "libgst_emu.so.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-DGST_CPU_EMU",
+          "-x", "c++", os.path.join(CSRC, "gst_kernels.hip"),
+          "-x", "c++", os.path.join(CSRC, "gst_api.cpp"), "-o", tmp], verbose)
+    os.replace(tmp, EMU_LIB)
     tmp = os.path.join(OUT, "libgstgen.so.tmp")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
           os.path.join(CSRC, "gen.cpp"), "-o", tmp], verbose)

@@ -1,0 +1,84 @@
+"""Runs small TrimCDP / D-TrimCDP cases through the CPU emulation build of the
+kernels (libgst_emu.so, selected with GPU4GST_LIB) and compares them with the
+oracle. Started as a child process by test_emu.py so that the emulation
+library is the only libgst loaded in that process. Prints one JSON summary.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, os.path.join(ROOT, "gpu4gst-sigmod-fullexp_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+from gpu4gst import GSTGraph, workloads, STATUS_OK, STATUS_INFEASIBLE  # noqa: E402
+import oracle as O  # noqa: E402
+
+
+def check_tree(g, grp, q, cost, edges, D=-1):
+    if len(edges) == 0:
+        gs = np.zeros(g.n, np.int64)
+        for i, gid in enumerate(q):
+            gs[grp.members(int(gid))] |= 1 << i
+        return cost == 0 and bool((gs == (1 << len(q)) - 1).any())
+    code, wsum = O.verify_tree(g, grp, q, edges, D)
+    return code == 0 and wsum == cost
+
+
+def main(n_plain, n_hop):
+    fails, cases = [], 0
+    for seed in range(n_plain):
+        rng = np.random.default_rng(5000 + seed)
+        n = int(rng.integers(4, 40))
+        g, grp = workloads.small(n, int(rng.integers(0, 3 * n)), 6, seed_=500 + seed,
+                                 whi=int(rng.choice([1, 7, 60])),
+                                 member_p=float(rng.choice([0.05, 0.2])))
+        G = GSTGraph.from_generated(g)
+        G.set_groups(grp)
+        qs = [rng.choice(6, k, replace=False).astype(np.int32) for k in (1, 3, 5, 6)]
+        for q in qs:
+            cases += 1
+            r = G.solve(q)
+            ref = O.dpbf(g, grp, q)
+            if ref["cost"] >= O.INF():
+                ok = r.status == STATUS_INFEASIBLE
+            else:
+                ok = r.status == STATUS_OK and r.cost == ref["cost"] and check_tree(
+                    g, grp, q, r.cost, r.edges)
+            if not ok:
+                fails.append(("plain", seed, q.tolist(), int(r.cost), int(ref["cost"])))
+        # the same queries in one lockstep batch
+        b = G.solve_batch(qs)
+        for i, q in enumerate(qs):
+            ref = O.dpbf(g, grp, q)
+            if ref["cost"] < O.INF() and int(b.costs[i]) != ref["cost"]:
+                fails.append(("batch", seed, q.tolist(), int(b.costs[i]), int(ref["cost"])))
+        G.close()
+    for seed in range(n_hop):
+        rng = np.random.default_rng(6000 + seed)
+        n = int(rng.integers(4, 30))
+        g, grp = workloads.small(n, int(rng.integers(0, 2 * n)), 5, seed_=600 + seed,
+                                 whi=int(rng.choice([1, 9])), member_p=0.2)
+        G = GSTGraph.from_generated(g)
+        G.set_groups(grp)
+        for D in (0, 1, 3):
+            q = rng.choice(5, int(rng.integers(1, 5)), replace=False).astype(np.int32)
+            cases += 1
+            r = G.solve(q, D=D)
+            ref = O.hop(g, grp, q, D)
+            if ref["cost"] >= O.INF():
+                ok = r.status == STATUS_INFEASIBLE
+            else:
+                ok = r.status == STATUS_OK and r.cost == ref["cost"] and check_tree(
+                    g, grp, q, r.cost, r.edges, D)
+            if not ok:
+                fails.append(("hop", seed, D, q.tolist(), int(r.cost), int(ref["cost"])))
+        G.close()
+    print(json.dumps({"cases": cases, "fails": fails}))
+
+
+if __name__ == "__main__":
+    main(int(sys.argv[1]), int(sys.argv[2]))

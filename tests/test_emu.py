@@ -1,0 +1,73 @@
+"""Kernel logic on the CPU: the HIP source built against the CPU emulation
+header (libgst_emu.so, built by __graft_entry__.build()) solves small plain
+and hop-bounded queries, single and batched, and must agree bit-exactly with
+the oracle; the C++ adapters with the reference solvers' call shapes
+(include/gst_reference_adapters.hpp) are compiled and driven the same way.
+"""
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from gpu4gst import workloads
+import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EMU = os.path.join(ROOT, "gpu4gst-sigmod-fullexp_amd", "_build", "libgst_emu.so")
+
+
+def _need_emu():
+    if not os.path.exists(EMU):
+        pytest.skip("libgst_emu.so not built (run __graft_entry__.build())")
+
+
+def test_emulated_kernels_match_oracle():
+    _need_emu()
+    env = dict(os.environ, GPU4GST_LIB=EMU)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu_cases.py"), "8", "5"],
+                       env=env, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["cases"] == 47
+    assert out["fails"] == [], out["fails"]
+
+
+def test_reference_shaped_adapters(tmp_path):
+    _need_emu()
+    exe = tmp_path / "adapter_demo"
+    cc = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+                         os.path.join(ROOT, "tests", "cpp", "adapter_demo.cpp"),
+                         EMU, f"-Wl,-rpath,{os.path.dirname(EMU)}", "-o", str(exe)],
+                        capture_output=True, text=True)
+    assert cc.returncode == 0, cc.stderr
+    g, grp = workloads.small(24, 50, 5, seed_=77, whi=9, member_p=0.2)
+    u, v, w = g.src, g.dst, g.w
+    lines = [f"{g.n} {len(u)}"] + [f"{a} {b} {c}" for a, b, c in zip(u, v, w)]
+    lines.append(str(grp.count))
+    for i in range(grp.count):
+        m = grp.members(i)
+        lines.append(" ".join(map(str, [len(m), *m])))
+    queries = [(-1, [0, 1, 2]), (-1, [3, 4]), (2, [0, 1, 2]), (1, [1, 3]), (-1, [0, 1, 2, 3, 4])]
+    lines.append(str(len(queries)))
+    for D, q in queries:
+        lines.append(" ".join(map(str, [D, len(q), *q])))
+    r = subprocess.run([str(exe)], input="\n".join(lines) + "\n", capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    got = [json.loads(s) for s in r.stdout.strip().splitlines()]
+    assert len(got) == len(queries)
+    for (D, q), res in zip(queries, got):
+        qa = np.array(q, np.int32)
+        ref = O.dpbf(g, grp, qa) if D < 0 else O.hop(g, grp, qa, D)
+        if ref["cost"] >= O.INF():
+            # the reference's conventions: 0 from DP_GPU, 100000 from DP_gpu
+            assert res["status"] == 1 and res["cost"] == (0 if D < 0 else 100000)
+            continue
+        assert res["status"] == 0 and res["cost"] == ref["cost"]
+        if res["edges"]:
+            e = np.array(res["edges"], np.int32)
+            code, wsum = O.verify_tree(g, grp, qa, e, D)
+            assert code == 0 and wsum == res["cost"]
