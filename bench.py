@@ -75,12 +75,156 @@ def cpu_baseline(w, budget_s: float, single_costs: dict):
             "cost_mismatches_vs_gpu": mismatches}
 
 
+def _apply_options(G, args):
+    if args.chunk:
+        G.set_option("chunk", args.chunk)
+    if args.sssp_delta >= 0:
+        G.set_option("sssp_delta", args.sssp_delta)
+    if args.dp_div >= 0:
+        G.set_option("dp_delta_div", args.dp_div)
+    if args.grid:
+        G.set_option("grid", args.grid)
+    if args.radius_pct:
+        G.set_option("radius_pct", args.radius_pct)
+    if args.max_batch:
+        G.set_option("max_batch", args.max_batch)
+
+
+def _profiled_roofline(G, w, D, hop: bool):
+    """Instrumented solve: every expand launch timed with HIP events on the
+    solver's stream; returns (dominant, other) roofline dicts."""
+    G.set_option("profile", 1)
+    pr = G.solve_batch(w.queries, D=D)
+    G.set_option("profile", 0)
+    ps = pr.stats
+    log(f"profiled: dp expand {ps['expand_ms']:.1f} ms / {ps['expand_launches']} launches, "
+        f"one-label expand {ps['sssp_expand_ms']:.1f} ms")
+    traffic = _traffic(w.name)
+    # one-label expand: item 8 + row 32 + row bounds 8 B; edge visit: col 4 + weight 4 +
+    # neighbour row 32 B; label update: 4 B
+    sssp_bytes = 48 * ps["sssp_items"] + 40 * ps["sssp_relax"] + 4 * ps["sssp_updates"]
+    roof_sssp = _roof("k_sssp_expand", sssp_bytes, ps["sssp_expand_ms"],
+                      max(1, ps["sssp_expand_launches"]), traffic.get("k_sssp_expand"))
+    kname = "k_dp_expand<true>" if hop else "k_dp_expand<false>"
+    roof_dp = _roof(kname, ps["bytes_alg"], ps["expand_ms"], max(1, ps["expand_launches"]),
+                    traffic.get(kname))
+    if ps["sssp_expand_ms"] >= ps["expand_ms"]:
+        return roof_sssp, roof_dp
+    return roof_dp, roof_sssp
+
+
+def _dp_stats(st):
+    return {k: st[k] for k in ("sssp_iterations", "sssp_items", "sssp_relax", "sssp_updates",
+                               "sssp_items_unbounded", "dp_iterations", "states_expanded",
+                               "grow_relax", "merge_relax", "improvements", "max_frontier",
+                               "batches")}
+
+
+def other_workload(args):
+    """Workloads 1, 3, 4, 5: batched throughput (hop-bounded for 3, forward +
+    backward through the autograd wrapper for 4), single-query sample, cost
+    parity, roofline of the dominant kernel and a CPU baseline sample."""
+    import torch
+    from gpu4gst import GSTGraph, workloads
+    t0 = time.perf_counter()
+    w = workloads.config(args.workload, n_queries=args.queries or None)
+    log(f"workload {w.name}: V={w.graph.n} E={w.graph.m} groups={w.groups.count} "
+        f"queries={len(w.queries)} ({time.perf_counter() - t0:.1f} s to generate)")
+    G = GSTGraph.from_generated(w.graph)
+    G.set_groups(w.groups)
+    _apply_options(G, args)
+    D = None if w.D is None else w.D
+    hop = D is not None
+    torch.cuda.synchronize()
+    if args.profile_only:
+        G.set_option("profile", 1)
+        ps = G.solve_batch(w.queries, D=D).stats
+        print(json.dumps({"profile_only": w.name, "expand_ms": ps["expand_ms"],
+                          "expand_launches": ps["expand_launches"],
+                          "sssp_expand_ms": ps["sssp_expand_ms"],
+                          "sssp_expand_launches": ps["sssp_expand_launches"],
+                          "phase_ms": ps["phase_ms"], "batches": ps["batches"]}))
+        return
+    grad = args.workload == 4
+    if grad:
+        from gpu4gst.autograd import gst_cost
+        wt = torch.tensor(w.graph.w.astype(np.float32), device="cuda:0", requires_grad=True)
+
+    def step():
+        if grad:
+            c = gst_cost(wt, G, w.queries)
+            wt.grad = None
+            c.sum().backward()
+            return c.detach().cpu().numpy().astype(np.int64)
+        return G.solve_batch(w.queries, D=D).costs
+
+    for i in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    costs = [step() for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    qps = args.steps * len(w.queries) / el
+    base = G.solve_batch(w.queries, D=D)
+    stable = all(np.array_equal(np.asarray(c, np.int64), base.costs.astype(np.int64))
+                 for c in costs)
+    st = base.stats
+    log(f"batched: {qps:.1f} queries/s, {el / args.steps * 1e3:.1f} ms/step")
+    ns = min(args.single_sample, len(w.queries))
+    if args.single_chunk:
+        G.set_option("chunk", args.single_chunk)
+    t1 = time.perf_counter()
+    single = {}
+    for i in range(ns):
+        single[i] = G.solve(w.queries[i], D=-1 if D is None else int(D[i])).cost
+    el1 = time.perf_counter() - t1
+    mism = sum(int(base.costs[i] != single[i]) for i in range(ns))
+    G.set_option("chunk", args.chunk or 4096)
+    dominant, other = _profiled_roofline(G, w, D, hop)
+    out = {
+        "metric": ("GST queries/s, forward + backward (batched, exact cost)" if grad else
+                   "GST queries/s (batched, exact cost" + (", hop-bounded)" if hop else ")")),
+        "value": qps, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": el / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": w.name, "V": w.graph.n, "E_undirected": w.graph.m,
+                   "groups": w.groups.count, "queries_per_step": len(w.queries),
+                   "k": sorted({len(q) for q in w.queries}),
+                   "hop_bound": None if D is None else sorted({int(x) for x in D})},
+        "phase_ms": st["phase_ms"], "single_query_qps": ns / el1, "single_sample": ns,
+        "batched_over_single": qps / (ns / el1),
+        "cost_mismatch_batched_vs_single": mism, "costs_stable_across_steps": stable,
+        "dp_stats": _dp_stats(st), "roofline": dominant, "roofline_other_kernel": other,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        t0 = time.perf_counter()
+        n, bad = 0, 0
+        for i, q in enumerate(w.queries):
+            r = O.hop(w.graph, w.groups, q, int(D[i])) if hop else O.prunedpp(
+                w.graph, w.groups, q)
+            bad += int(r["cost"] != int(base.costs[i]))
+            n += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        el2 = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n / el2, "unit": "queries/s", "cores": 1, "kind": "port",
+            "sample": f"first {n} queries, restated {'HOP_cpu' if hop else 'PrunedDP++'}, "
+                      f"{el2:.1f} s", "cost_mismatches_vs_gpu": bad}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--queries", type=int, default=1024)
+    ap.add_argument("--queries", type=int, default=0,
+                    help="queries per step (0 = the workload's own count)")
     ap.add_argument("--single-sample", type=int, default=96)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -99,6 +243,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration) and exit")
     ap.add_argument("--single-dp-div", type=int, default=-1)
+    ap.add_argument("--workload", type=int, default=2,
+                    help="2 = the headline workload (default); 1, 3, 4, 5 = the other seeded "
+                         "workloads (3 hop-bounded, 4 forward + backward, 5 R-MAT scale 22)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -117,6 +264,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_mib": args.copy_probe_mib}))
         return
+    if args.workload != 2:
+        other_workload(args)
+        return
     from gpu4gst import GSTGraph, workloads
     t0 = time.perf_counter()
     w = workloads.config(2, n_queries=args.queries)
@@ -125,18 +275,7 @@ def main():
     assert torch.cuda.is_available()
     G = GSTGraph.from_generated(w.graph)
     G.set_groups(w.groups)
-    if args.chunk:
-        G.set_option("chunk", args.chunk)
-    if args.sssp_delta >= 0:
-        G.set_option("sssp_delta", args.sssp_delta)
-    if args.dp_div >= 0:
-        G.set_option("dp_delta_div", args.dp_div)
-    if args.grid:
-        G.set_option("grid", args.grid)
-    if args.radius_pct:
-        G.set_option("radius_pct", args.radius_pct)
-    if args.max_batch:
-        G.set_option("max_batch", args.max_batch)
+    _apply_options(G, args)
     torch.cuda.synchronize()
 
     if args.profile_only:
@@ -187,25 +326,8 @@ def main():
         G.set_option("chunk", args.chunk or 4096)
     if args.single_dp_div >= 0:
         G.set_option("dp_delta_div", args.dp_div if args.dp_div >= 0 else 0)
-    # instrumented pass: every expand launch timed with HIP events on the solver's stream
-    G.set_option("profile", 1)
-    pr = G.solve_batch(w.queries)
-    G.set_option("profile", 0)
-    ps = pr.stats
-    log(f"profiled: dp expand {ps['expand_ms']:.1f} ms / {ps['expand_launches']} launches, "
-        f"one-label expand {ps['sssp_expand_ms']:.1f} ms")
+    dominant, other = _profiled_roofline(G, w, None, False)
     relax = st["grow_relax"] + st["merge_relax"]
-    traffic = _traffic(w.name)
-    # one-label expand: item 8 + row 32 + row bounds 8 B; edge visit: col 4 + weight 4 +
-    # neighbour row 32 B; label update: 4 B
-    sssp_bytes = 48 * ps["sssp_items"] + 40 * ps["sssp_relax"] + 4 * ps["sssp_updates"]
-    sssp_launches = max(1, ps["sssp_expand_launches"])
-    roof_sssp = _roof("k_sssp_expand", sssp_bytes, ps["sssp_expand_ms"], sssp_launches,
-                      traffic.get("k_sssp_expand"))
-    roof_dp = _roof("k_dp_expand<false>", ps["bytes_alg"], ps["expand_ms"],
-                    max(1, ps["expand_launches"]), traffic.get("k_dp_expand<false>"))
-    dominant, other = (roof_sssp, roof_dp) if ps["sssp_expand_ms"] >= ps["expand_ms"] else (
-        roof_dp, roof_sssp)
 
     out = {
         "metric": "GST queries/s at k=3/5/7 (batched, exact cost)",
@@ -231,11 +353,7 @@ def main():
         "cost_mismatch_batched_vs_single": mism_single,
         "costs_stable_across_steps": stable,
         "state_relaxations_per_s": relax * args.steps / el,
-        "dp_stats": {k: st[k] for k in ("sssp_iterations", "sssp_items", "sssp_relax",
-                                         "sssp_updates", "sssp_items_unbounded",
-                                         "dp_iterations", "states_expanded",
-                                         "grow_relax", "merge_relax", "improvements",
-                                         "max_frontier", "batches")},
+        "dp_stats": _dp_stats(st),
         "roofline": dominant,
         "roofline_other_kernel": other,
     }

@@ -2,6 +2,7 @@
 
 libgst.so    HIP kernels + C ABI for gfx950 (hipcc), loaded by gpu4gst._lib
 libgstgen.so seeded workload generators (host only, g++)
+gst_run      the reference harnesses' command lines over libgst.so (host only, g++)
 
 Every call recompiles from source; objects go to _build/ (git-ignored).
 """
@@ -51,6 +52,11 @@ def build(verbose: bool = False) -> None:
           "-x", "c++", os.path.join(CSRC, "gst_kernels.hip"),
           "-x", "c++", os.path.join(CSRC, "gst_api.cpp"), "-o", tmp], verbose)
     os.replace(tmp, EMU_LIB)
+    INC = os.path.join(os.path.dirname(HERE), "include")
+    tmp = os.path.join(OUT, "gst_run.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, os.path.join(CSRC, "gst_cli.cpp"),
+          os.path.join(OUT, "libgst.so"), "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, os.path.join(OUT, "gst_run"))
     tmp = os.path.join(OUT, "libgstgen.so.tmp")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
           os.path.join(CSRC, "gen.cpp"), "-o", tmp], verbose)

@@ -96,3 +96,20 @@ def small(n: int, m: int, G: int, *, seed_: int, wlo=1, whi=20, member_p=0.2):
             mem = [int(rng.integers(0, n))]
         lists.append(mem)
     return g, gen.groups_from_lists(lists)
+
+
+def degree_sequence_graph(degrees, *, seed_: int, wlo: int = 1, whi: int = 100) -> gen.Graph:
+    """Configuration-model graph on a given degree sequence (stubs paired
+    uniformly at random; loops and repeated pairs dropped), vertex ids kept,
+    weights U[wlo, whi]. Used with the reference's Twitch_beg_pos.bin, whose
+    edge and weight files are not shipped, to get a graph of Twitch's exact
+    vertex count and (up to the dropped pairs) degree sequence."""
+    deg = np.asarray(degrees, np.int64)
+    rng = np.random.default_rng(seed_)
+    stubs = np.repeat(np.arange(deg.shape[0], dtype=np.int64), deg)
+    rng.shuffle(stubs)
+    stubs = stubs[: stubs.shape[0] // 2 * 2].reshape(-1, 2)
+    keep = stubs[:, 0] != stubs[:, 1]
+    src, dst = stubs[keep, 0], stubs[keep, 1]
+    w = rng.integers(wlo, whi + 1, src.shape[0])
+    return gen.from_edges(int(deg.shape[0]), src, dst, w, lcc=False)
