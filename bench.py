@@ -8,9 +8,15 @@ no hop bound. A step is one solve of all 1,024 queries in one batched call
 
 Printed JSON line: queries/s of the batched path (value), the single-query
 path on a sample of the same queries, cost parity between the two, the
-roofline of the DP expand kernel (algorithmic bytes / measured kernel time,
-from an instrumented pass timed with HIP events on the solver's stream) and
-the CPU baseline (restated PrunedDP++, one thread, on a bounded sample).
+roofline of the dominant expand kernel (algorithmic bytes / measured kernel
+time, from an instrumented pass timed with HIP events on the solver's stream)
+and the CPU baselines on bounded samples: restated TrimCDP-multi-core-CPU on
+the job's host threads (cpu_baseline) and restated PrunedDP++ on one thread
+(cpu_baseline_one_core).
+
+--workload 1/3/4/5 measures the other seeded workloads the same way
+(3: hop-bounded, against restated HOP_MT / HOP_cpu; 4: forward + backward
+through the autograd wrapper; 5: R-MAT scale 22, k = 7).
 """
 from __future__ import annotations
 
@@ -53,26 +59,55 @@ def _roof(kernel, bytes_alg, kernel_ms, launches, traffic):
             "kernel_ms_per_launch": kernel_ms / launches}
 
 
-def cpu_baseline(w, budget_s: float, single_costs: dict):
-    """Restated PrunedDP++ (oracle, one thread) over the first queries until
-    budget_s seconds of CPU work; costs are checked against the GPU's."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
+def cpu_threads() -> int:
+    """Host threads for the multi-core baselines: the CPU share the job was
+    given (OMP_NUM_THREADS) or the visible cores, at most 100 as in the
+    reference's pool (code/TrimCDP-multi-core-CPU/include/CPUNONHOP.h:692)."""
+    n = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
+    return max(1, min(n, 100))
+
+
+def cpu_sample(w, budget_s: float, gpu_costs, solve, name: str, cores: int):
+    """Times `solve` (an oracle function) over the first queries of the
+    workload until budget_s seconds have passed; costs are checked against
+    the GPU's."""
     t0 = time.perf_counter()
     n, mismatches = 0, 0
     for i, q in enumerate(w.queries):
-        r = O.prunedpp(w.graph, w.groups, q)
+        r = solve(i, q)
         n += 1
-        if i in single_costs and r["cost"] != single_costs[i]:
+        if i < len(gpu_costs) and r["cost"] != int(gpu_costs[i]):
             mismatches += 1
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
     ks = sorted({len(q) for q in w.queries[:n]})
-    return {"value": n / el, "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} queries (k in {ks}) of the same workload, restated PrunedDP++ "
-                      f"(binary heap, array tables), {el:.1f} s",
+    return {"value": n / el, "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} queries (k in {ks}) of the same workload, {name}, {el:.1f} s",
             "cost_mismatches_vs_gpu": mismatches}
+
+
+def cpu_baselines(w, budget_s: float, gpu_costs, D=None):
+    """(multi-core baseline, one-core baseline): restated TrimCDP-multi-core-CPU
+    cpu_mt and PrunedDP++ for plain queries, restated D-TrimCDP-multi-core-CPU
+    HOP_MT and D-PrunedDP++ HOP_cpu for hop-bounded ones."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t = cpu_threads()
+    if D is None:
+        mt = cpu_sample(w, budget_s, gpu_costs,
+                        lambda i, q: O.trimcdp_mt(w.graph, w.groups, q, t),
+                        f"restated TrimCDP-multi-core-CPU ({t} threads)", t)
+        one = cpu_sample(w, budget_s, gpu_costs, lambda i, q: O.prunedpp(w.graph, w.groups, q),
+                         "restated PrunedDP++ (binary heap, array tables, 1 thread)", 1)
+    else:
+        mt = cpu_sample(w, budget_s, gpu_costs,
+                        lambda i, q: O.hop_mt(w.graph, w.groups, q, int(D[i]), t),
+                        f"restated D-TrimCDP-multi-core-CPU HOP_MT ({t} threads)", t)
+        one = cpu_sample(w, budget_s, gpu_costs,
+                         lambda i, q: O.hop(w.graph, w.groups, q, int(D[i])),
+                         "restated D-PrunedDP++ HOP_cpu (1 thread)", 1)
+    return mt, one
 
 
 def _apply_options(G, args):
@@ -88,6 +123,8 @@ def _apply_options(G, args):
         G.set_option("radius_pct", args.radius_pct)
     if args.max_batch:
         G.set_option("max_batch", args.max_batch)
+    if args.sssp_slots >= 0:
+        G.set_option("sssp_slots", args.sssp_slots)
 
 
 def _profiled_roofline(G, w, D, hop: bool):
@@ -199,22 +236,8 @@ def other_workload(args):
         "dp_stats": _dp_stats(st), "roofline": dominant, "roofline_other_kernel": other,
     }
     if not args.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        t0 = time.perf_counter()
-        n, bad = 0, 0
-        for i, q in enumerate(w.queries):
-            r = O.hop(w.graph, w.groups, q, int(D[i])) if hop else O.prunedpp(
-                w.graph, w.groups, q)
-            bad += int(r["cost"] != int(base.costs[i]))
-            n += 1
-            if time.perf_counter() - t0 > args.cpu_seconds:
-                break
-        el2 = time.perf_counter() - t0
-        out["cpu_baseline"] = {
-            "value": n / el2, "unit": "queries/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} queries, restated {'HOP_cpu' if hop else 'PrunedDP++'}, "
-                      f"{el2:.1f} s", "cost_mismatches_vs_gpu": bad}
+        out["cpu_baseline"], out["cpu_baseline_one_core"] = cpu_baselines(
+            w, args.cpu_seconds, base.costs, D)
     print(json.dumps(out), flush=True)
 
 
@@ -226,7 +249,8 @@ def main():
     ap.add_argument("--queries", type=int, default=0,
                     help="queries per step (0 = the workload's own count)")
     ap.add_argument("--single-sample", type=int, default=96)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="time budget of each CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--sssp-delta", type=int, default=-1)
@@ -235,6 +259,8 @@ def main():
     ap.add_argument("--radius-pct", type=int, default=0)
     ap.add_argument("--max-batch", type=int, default=0,
                     help="cap on queries per lockstep batch (0 = as many as the HBM budget admits)")
+    ap.add_argument("--sssp-slots", type=int, default=-1,
+                    help="queries per one-label pass inside a batch (0 = all at once)")
     ap.add_argument("--single-chunk", type=int, default=256,
                     help="virtual-split chunk for the single-query path (its best setting; "
                          "0 = same as batched)")
@@ -358,7 +384,8 @@ def main():
         "roofline_other_kernel": other,
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds, single)
+        out["cpu_baseline"], out["cpu_baseline_one_core"] = cpu_baselines(
+            w, args.cpu_seconds, base.costs)
     print(json.dumps(out), flush=True)
 
 

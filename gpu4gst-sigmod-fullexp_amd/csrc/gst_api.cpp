@@ -107,6 +107,7 @@ struct gst_graph {
   int32_t dp_div = 0;
   int32_t radius_pct = 100;
   int32_t max_batch = 0;
+  int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
   int32_t maxw = 1;
   // engine buffers (grow-only)
   DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
@@ -365,13 +366,23 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   const Task* dt = g->tasks.as<Task>();
   const int nt = (int)tasks.size();
   uint32_t* cnt = g->cnt.as<uint32_t>();
-  launch_groups_setup(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(),
-                      g->wl0.as<uint64_t>(), cnt, g->stream);
-  HIP_TRY(hipGetLastError());
   double sssp_ms = 0, dp_ms = 0;
   int64_t dp_launches = 0, sssp_launches = 0;
-  int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
-  if (rc) return rc;  // includes 100 = overflow
+  // The one-label phase runs over groups of sssp_slots slots at a time (tasks
+  // are in slot order), so that the label rows one pass touches can stay in
+  // the last-level cache; 0 = all slots in one pass.
+  const int per_pass = g->sssp_slots > 0 ? g->sssp_slots : nslots;
+  for (int t0 = 0; t0 < nt;) {
+    const int s_end = std::min(nslots, tasks[t0].slot + per_pass);
+    int t1 = t0;
+    while (t1 < nt && tasks[t1].slot < s_end) ++t1;
+    launch_groups_setup(c, dt + t0, t1 - t0, g->goff.as<int64_t>(), g->gmem.as<int32_t>(),
+                        g->wl0.as<uint64_t>(), cnt, g->stream);
+    HIP_TRY(hipGetLastError());
+    int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
+    if (rc) return rc;  // includes 100 = overflow
+    t0 = t1;
+  }
   HIP_TRY(hipEventRecord(g->ph[1], g->stream));
   launch_clamp(c, nslots, g->stream);
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
@@ -384,7 +395,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   launch_dp_sched_init(c, nslots, g->stream);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipEventRecord(g->ph[2], g->stream));
-  rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
+  int rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(g->ph[3], g->stream));
   launch_backtrack(c, nslots, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
@@ -713,6 +724,10 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
     case GST_OPT_SSSP_DELTA:
       if (v < 0 || v > INF / 2) return fail(GST_ERR_INVALID, "band width out of range");
       g->sssp_delta = (int32_t)v;
+      break;
+    case GST_OPT_SSSP_SLOTS:
+      if (v < 0 || v > (int64_t)MAX_SLOTS) return fail(GST_ERR_INVALID, "slot group out of range");
+      g->sssp_slots = (int32_t)v;
       break;
     case GST_OPT_MAX_BATCH:
       if (v < 0 || v > (int64_t)MAX_SLOTS) return fail(GST_ERR_INVALID, "max batch out of range");

@@ -101,6 +101,7 @@ typedef struct gst_stats {
                                       distances beyond it are clamped lower bounds */
 #define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */
 #define GST_OPT_DP_DELTA_DIV 11    /* DP cost band = upper bound / value (default 0 = unordered) */
+#define GST_OPT_SSSP_SLOTS 15      /* queries per one-label pass inside a batch (0 = all at once) */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
  * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge

@@ -20,6 +20,8 @@
 //   orc_trimcdp_mt TrimCDP-multi-core-CPU cpu_mt
 //                  (code/TrimCDP-multi-core-CPU/include/CPUNONHOP.h:573-842), a
 //                  level-synchronous label-correcting DP over a thread pool.
+//   orc_hop_mt     D-TrimCDP-multi-core-CPU HOP_MT
+//                  (code/D-TrimCDP-multi-core-CPU/include/CPUHOP.h:166-392).
 //   orc_dpbf       textbook DPBF (Ding et al., ICDE'07): Dijkstra over (v, p)
 //                  without pruning; exact.
 //   orc_brute_*    exhaustive search on tiny graphs (vertex subsets + MST for the
@@ -31,7 +33,8 @@
 //
 // Parity status of each function (how it is pinned) is recorded in the test
 // suite: orc_dpbf and orc_hop against brute force and closed forms on random
-// tiny graphs; orc_prunedpp and orc_trimcdp_mt against orc_dpbf.
+// tiny graphs; orc_prunedpp and orc_trimcdp_mt against orc_dpbf; orc_hop_mt
+// against orc_hop and brute force.
 //
 // Conventions: CSR int64 beg[V+1], int32 col/w (symmetric, positive weights);
 // groups CSR int64 goff[G+1], int32 gmem; a query is k group ids, bit i of a
@@ -741,6 +744,107 @@ int orc_trimcdp_mt(int32_t V, const int64_t* beg, const int32_t* col, const int3
   for (int32_t v = 0; v < V; ++v) fin = std::min(fin, cost[(int64_t)v * width + full].load());
   if (dp_seconds) *dp_seconds = now_s() - tloop;
   if (total_seconds) *total_seconds = now_s() - t0;
+  if (stats) {
+    stats[0] = process;
+    stats[1] = levels;
+  }
+  if (out_cost) *out_cost = fin;
+  return fin >= INF ? INFEASIBLE : OK;
+}
+
+// Restated D-TrimCDP-multi-core-CPU HOP_MT
+// (code/D-TrimCDP-multi-core-CPU/include/CPUHOP.h:166-392): level-synchronous
+// label-correcting DP over (v, p, d) with atomic costs and no pruning beyond
+// "cost >= best" (the reference caps its pool at 50 threads, CPUHOP.h:70).
+// Seeds are single bits at d = 0 (set_max_ID_mt, CPUHOP.h:124-136); a full
+// state updates best when it is processed; grow (v,p,d) -> (u,p,d+1) for
+// d < D; merge with every p2 disjoint from p and d2 <= D - d into
+// (v, p|p2, max(d, d2)). stats[0] = states processed, stats[1] = levels.
+int orc_hop_mt(int32_t V, const int64_t* beg, const int32_t* col, const int32_t* w, int32_t G,
+               const int64_t* goff, const int32_t* gmem, int32_t k, const int32_t* q, int32_t D,
+               int32_t threads, int32_t* out_cost, double* dp_seconds, int64_t* stats) {
+  Graph g{V, beg, col, w};
+  Query Q;
+  if (k >= 15 || D < 0 || !make_query(g, goff, gmem, G, k, q, Q)) return INVALID;
+  if (threads <= 0)
+    threads = (int32_t)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 50u);
+  const int32_t full = Q.full, width = 1 << k, L = D + 1;
+  auto at = [&](int32_t v, int32_t p, int32_t d) { return ((int64_t)v * width + p) * L + d; };
+  std::vector<std::atomic<int32_t>> cost((size_t)V * width * L);
+  for (auto& x : cost) x.store(INF, std::memory_order_relaxed);
+  auto fetch_min = [](std::atomic<int32_t>& a, int32_t v) {
+    int32_t cur = a.load(std::memory_order_relaxed);
+    while (v < cur && !a.compare_exchange_weak(cur, v, std::memory_order_relaxed)) {
+    }
+  };
+  struct Item {
+    int32_t v, p, d, c;
+  };
+  std::vector<Item> cur;
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t i = 0; i < k; ++i)
+      if (Q.gs[v] & (1 << i)) {
+        cost[at(v, 1 << i, 0)].store(0, std::memory_order_relaxed);
+        cur.push_back({v, 1 << i, 0, 0});
+      }
+  std::atomic<int32_t> best(INF);
+  int64_t process = 0, levels = 0;
+  const double t0 = now_s();
+  std::vector<std::vector<Item>> local(threads);
+  while (!cur.empty()) {
+    process += (int64_t)cur.size();
+    ++levels;
+    std::atomic<int64_t> next_item(0);
+    auto worker = [&](int32_t tid) {
+      auto& out = local[tid];
+      out.clear();
+      while (true) {
+        const int64_t it = next_item.fetch_add(1, std::memory_order_relaxed);
+        if (it >= (int64_t)cur.size()) break;
+        const Item x = cur[it];
+        if (x.c != cost[at(x.v, x.p, x.d)].load(std::memory_order_relaxed)) continue;  // stale
+        const int32_t lbest = best.load(std::memory_order_relaxed);
+        if (x.p == full) {
+          if (x.c < lbest) fetch_min(best, x.c);
+          continue;
+        }
+        if (x.c >= lbest) continue;
+        if (x.d < D) {
+          for (int64_t j = beg[x.v]; j < beg[x.v + 1]; ++j) {
+            const int32_t u = col[j], nc = x.c + w[j];
+            auto& cu = cost[at(u, x.p, x.d + 1)];
+            if (nc < cu.load(std::memory_order_relaxed)) {
+              fetch_min(cu, nc);
+              out.push_back({u, x.p, x.d + 1, nc});
+            }
+          }
+        }
+        for (int32_t p2 = 1; p2 <= full; ++p2) {
+          if (p2 & x.p) continue;
+          for (int32_t d2 = 0; d2 <= D - x.d; ++d2) {
+            const int32_t c2 = cost[at(x.v, p2, d2)].load(std::memory_order_relaxed);
+            if (c2 == INF) continue;
+            const int32_t P = x.p | p2, nd = std::max(x.d, d2), m = x.c + c2;
+            auto& cP = cost[at(x.v, P, nd)];
+            if (m < cP.load(std::memory_order_relaxed)) {
+              fetch_min(cP, m);
+              out.push_back({x.v, P, nd, m});
+            }
+          }
+        }
+      }
+    };
+    std::vector<std::thread> pool;
+    for (int32_t t = 1; t < threads; ++t) pool.emplace_back(worker, t);
+    worker(0);
+    for (auto& th : pool) th.join();
+    cur.clear();
+    for (auto& l : local) cur.insert(cur.end(), l.begin(), l.end());
+  }
+  int32_t fin = best.load();
+  for (int32_t v = 0; v < V; ++v)
+    for (int32_t d = 0; d <= D; ++d) fin = std::min(fin, cost[at(v, full, d)].load());
+  if (dp_seconds) *dp_seconds = now_s() - t0;
   if (stats) {
     stats[0] = process;
     stats[1] = levels;

@@ -38,6 +38,7 @@ def _lib():
         L.orc_dpbf.argtypes = common + [vp, vp, i32, vp]
         L.orc_hop.argtypes = common + [i32, vp, vp, i32, vp, vp, vp]
         L.orc_trimcdp_mt.argtypes = common + [i32, vp, vp, vp, vp]
+        L.orc_hop_mt.argtypes = common + [i32, i32, vp, vp, vp]
         L.orc_brute_plain.argtypes = common + [vp]
         L.orc_brute_hop.argtypes = common + [i32, vp]
         L.orc_verify_tree.argtypes = common + [i32, i32, vp, i32, vp]
@@ -108,6 +109,16 @@ def trimcdp_mt(g, groups, q, threads: int = 0):
     rc = _lib().orc_trimcdp_mt(*args, threads, C.byref(cost), C.byref(dp), C.byref(tot), _p(st))
     return dict(status=rc, cost=cost.value, dp_s=dp.value, total_s=tot.value,
                 processed=int(st[0]), levels=int(st[1]))
+
+
+def hop_mt(g, groups, q, D: int, threads: int = 0):
+    args, q = _c(g, groups, q)
+    cost = C.c_int32(0)
+    dp = C.c_double(0)
+    st = np.zeros(2, np.int64)
+    rc = _lib().orc_hop_mt(*args, D, threads, C.byref(cost), C.byref(dp), _p(st))
+    return dict(status=rc, cost=cost.value, dp_s=dp.value, processed=int(st[0]),
+                levels=int(st[1]))
 
 
 def brute_plain(g, groups, q):

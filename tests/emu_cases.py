@@ -50,12 +50,17 @@ def main(n_plain, n_hop):
                     g, grp, q, r.cost, r.edges)
             if not ok:
                 fails.append(("plain", seed, q.tolist(), int(r.cost), int(ref["cost"])))
-        # the same queries in one lockstep batch
-        b = G.solve_batch(qs)
-        for i, q in enumerate(qs):
-            ref = O.dpbf(g, grp, q)
-            if ref["cost"] < O.INF() and int(b.costs[i]) != ref["cost"]:
-                fails.append(("batch", seed, q.tolist(), int(b.costs[i]), int(ref["cost"])))
+        # the same queries in one lockstep batch, with the one-label phase in one
+        # pass and in passes of two queries
+        for per_pass in (0, 2):
+            G.set_option("sssp_slots", per_pass)
+            b = G.solve_batch(qs)
+            for i, q in enumerate(qs):
+                ref = O.dpbf(g, grp, q)
+                if ref["cost"] < O.INF() and int(b.costs[i]) != ref["cost"]:
+                    fails.append(("batch", per_pass, seed, q.tolist(), int(b.costs[i]),
+                                  int(ref["cost"])))
+        G.set_option("sssp_slots", 0)
         G.close()
     for seed in range(n_hop):
         rng = np.random.default_rng(6000 + seed)

@@ -1,7 +1,7 @@
 """The CPU oracle pinned against brute force, closed forms and invariants.
 
-orc_dpbf / orc_prunedpp / orc_trimcdp_mt (plain) and orc_hop (hop-bounded)
-must equal exhaustive search on tiny graphs; k = 1 and k = 2 reduce to
+orc_dpbf / orc_prunedpp / orc_trimcdp_mt (plain) and orc_hop / orc_hop_mt
+(hop-bounded) must equal exhaustive search on tiny graphs; k = 1 and k = 2 reduce to
 textbook answers; the hop optimum is non-increasing in D and equals the plain
 optimum once D >= V - 1. The tree verifier must reject planted mistakes.
 """
@@ -58,7 +58,9 @@ def test_hop_matches_brute_force():
             rb, cb = O.brute_hop(g, grp, q, D)
             r = O.hop(g, grp, q, D)
             assert r["status"] == rb
+            assert O.hop_mt(g, grp, q, D, 2)["status"] == rb
             if rb == O.OK:
+                assert O.hop_mt(g, grp, q, D, 2)["cost"] == cb
                 assert r["cost"] == cb
                 if len(r["edges"]):
                     code, wsum = O.verify_tree(g, grp, q, r["edges"], D)
@@ -129,6 +131,9 @@ def test_medium_graphs_all_oracles_agree():
             a = O.dpbf(g, grp, q)["cost"]
             assert O.prunedpp(g, grp, q)["cost"] == a
             assert O.trimcdp_mt(g, grp, q, 4)["cost"] == a
+            for D in (2, 4):
+                h = O.hop(g, grp, q, D)
+                assert O.hop_mt(g, grp, q, D, 4)["cost"] == h["cost"]
 
 
 def test_verifier_rejects_planted_mistakes():
