@@ -18,8 +18,13 @@
 #ifndef GST_REFERENCE_ADAPTERS_HPP
 #define GST_REFERENCE_ADAPTERS_HPP
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
+#include <functional>
+#include <map>
+#include <queue>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -138,6 +143,113 @@ tree dp_gpu_hop(CSR& /*graph*/, std::vector<unsigned int>& query, adjacency& gro
   ret.process_queue_num = (int)st.items_processed;
   ret.counts = (int)st.improvements;
   return t;
+}
+
+// Weight of the edge (a, b) in an adjacency (the smallest of parallel
+// copies), or -1 when there is none.
+inline int edge_weight(const adjacency& g, int a, int b) {
+  int w = -1;
+  if (a < 0 || (size_t)a >= g.size()) return -1;
+  for (const auto& e : g[a])
+    if (e.first == b && (w < 0 || e.second < w)) w = e.second;
+  return w;
+}
+
+// Sum of the tree's edge weights (graph_hash_of_mixed_weighted_sum_of_ec,
+// the cost the reference's CPU harness reports, code/PrunedDP++/include/
+// exp_CPU_nonHop.h:117); -1 if an edge is not in the graph.
+inline long long tree_cost(const tree& t, const adjacency& input_graph) {
+  long long c = 0;
+  for (const auto& e : t.edges) {
+    const int w = edge_weight(input_graph, e.first, e.second);
+    if (w < 0) return -1;
+    c += w;
+  }
+  return c;
+}
+
+// The checks of this_is_a_feasible_solution_gpu (code/PrunedDP++/include/
+// exp_CPU_nonHop.h:17-49): the tree's vertices form one connected component
+// and every query group has a member among them. A tree without edges is the
+// single vertex `root` (pass the vertex that covers every group; -1 = none).
+// On failure *why holds the reference's message.
+template <class Q>
+bool feasible(const tree& t, const adjacency& group_graph, const std::vector<Q>& query,
+              int root = -1, std::string* why = nullptr) {
+  std::map<int, std::vector<int>> adj;
+  for (const auto& e : t.edges) {
+    adj[e.first].push_back(e.second);
+    adj[e.second].push_back(e.first);
+  }
+  if (adj.empty() && root >= 0) adj[root];
+  if (adj.empty()) {
+    if (why) *why = "this_is_a_feasible_solution: solu is disconnected!";
+    return false;
+  }
+  std::set<int> seen{adj.begin()->first};
+  std::vector<int> st{adj.begin()->first};
+  while (!st.empty()) {
+    const int v = st.back();
+    st.pop_back();
+    for (int u : adj[v])
+      if (seen.insert(u).second) st.push_back(u);
+  }
+  if (seen.size() != adj.size()) {
+    if (why) *why = "this_is_a_feasible_solution: solu is disconnected!";
+    return false;
+  }
+  for (const auto& gq : query) {
+    const size_t gi = (size_t)gq;
+    bool covered = false;
+    if (gi < group_graph.size())
+      for (const auto& m : group_graph[gi])
+        if (seen.count(m.first)) {
+          covered = true;
+          break;
+        }
+    if (!covered) {
+      if (why) *why = "this_is_a_feasible_solution: a group is not covered!";
+      return false;
+    }
+  }
+  return true;
+}
+
+// MST post-processing (graph_hash_of_mixed_weighted_MST_postprocessing,
+// code/PrunedDP++/include/graph_hash_of_mixed_weighted_MST_postprocessing.h:
+// 38-66): the minimum spanning tree of the subgraph of input_graph induced by
+// the tree's vertices (Prim from the smallest vertex id). It never costs more
+// than the tree; for an optimal tree it costs the same.
+inline tree mst_postprocess(const tree& t, const adjacency& input_graph) {
+  std::set<int> verts;
+  for (const auto& e : t.edges) {
+    verts.insert(e.first);
+    verts.insert(e.second);
+  }
+  tree out;
+  out.status = t.status;
+  if (verts.empty()) return out;
+  typedef std::pair<long long, std::pair<int, int>> item;  // (w, (to, from))
+  std::priority_queue<item, std::vector<item>, std::greater<item>> pq;
+  std::set<int> in;
+  const int start = *verts.begin();
+  pq.push({0, {start, start}});
+  long long cost = 0;
+  while (!pq.empty() && in.size() < verts.size()) {
+    const item x = pq.top();
+    pq.pop();
+    const int v = x.second.first;
+    if (in.count(v)) continue;
+    in.insert(v);
+    if (v != x.second.second) {
+      out.edges.push_back({x.second.second, v});
+      cost += x.first;
+    }
+    for (const auto& e : input_graph[v])
+      if (verts.count(e.first) && !in.count(e.first)) pq.push({e.second, {e.first, v}});
+  }
+  out.cost = (int32_t)cost;
+  return out;
 }
 
 }  // namespace gst_ref

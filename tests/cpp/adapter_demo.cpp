@@ -5,6 +5,7 @@
 // Input on stdin: V M, M lines "u v w", G, G lines "n v1 .. vn", Q, Q lines "D k g1 .. gk".
 #include <cstdio>
 #include <iostream>
+#include <string>
 #include <vector>
 
 #include "gst_reference_adapters.hpp"
@@ -25,7 +26,43 @@ struct nogs_stand_in {
   int length = 0;
 };
 
-int main() {
+int selftest() {
+  // 4-cycle 0-1 (5), 1-2 (1), 2-3 (1), 3-0 (1); groups {0} and {3}, {1}
+  gst_ref::adjacency g(4), groups(3);
+  auto add = [&](int a, int b, int w) {
+    g[a].push_back({b, w});
+    g[b].push_back({a, w});
+  };
+  add(0, 1, 5);
+  add(1, 2, 1);
+  add(2, 3, 1);
+  add(3, 0, 1);
+  groups[0] = {{0, 1}};
+  groups[1] = {{3, 1}};
+  groups[2] = {{1, 1}};
+  std::vector<int> q{0, 1, 2};
+  gst_ref::tree path;
+  path.edges = {{0, 1}, {1, 2}, {2, 3}};
+  std::string why;
+  std::printf("selftest path_feasible %d\n", (int)gst_ref::feasible(path, groups, q, -1, &why));
+  std::printf("selftest path_cost %lld\n", gst_ref::tree_cost(path, g));
+  std::printf("selftest path_mst_cost %d\n", gst_ref::mst_postprocess(path, g).cost);
+  gst_ref::tree split;
+  split.edges = {{0, 1}, {2, 3}};
+  std::printf("selftest split_feasible %d\n", (int)gst_ref::feasible(split, groups, q, -1, &why));
+  std::printf("selftest split_why %s\n", why.find("disconnected") != std::string::npos ? "disconnected" : "other");
+  gst_ref::tree part;
+  part.edges = {{1, 2}, {2, 3}};
+  std::printf("selftest part_feasible %d\n", (int)gst_ref::feasible(part, groups, q, -1, &why));
+  std::printf("selftest part_why %s\n", why.find("not covered") != std::string::npos ? "uncovered" : "other");
+  gst_ref::tree single;
+  std::vector<int> q1{2};
+  std::printf("selftest single_feasible %d\n", (int)gst_ref::feasible(single, groups, q1, 1, &why));
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  if (argc > 1 && std::string(argv[1]) == "selftest") return selftest();
   int V, M;
   std::cin >> V >> M;
   gst_ref::adjacency input_graph(V);
@@ -70,7 +107,22 @@ int main() {
       long long ram = 0;
       t = gst_ref::dp_gpu_hop(csr, uq, group_graph, input_graph, D, &rt, cost, ram, ret, 0);
     }
-    std::printf("{\"D\": %d, \"cost\": %d, \"status\": %d, \"edges\": [", D, cost, t.status);
+    // a tree without edges is a vertex in every query group
+    int root = -1;
+    for (int v = 0; v < V && root < 0 && t.edges.empty(); ++v) {
+      bool all = true;
+      for (int gq : query) {
+        bool in = false;
+        for (const auto& m : group_graph[gq]) in |= m.first == v;
+        all &= in;
+      }
+      if (all) root = v;
+    }
+    const int ok = t.status == GST_Q_OK ? (int)gst_ref::feasible(t, group_graph, query, root) : 0;
+    std::printf("{\"D\": %d, \"cost\": %d, \"status\": %d, \"feasible\": %d, "
+                "\"tree_cost\": %lld, \"mst_cost\": %d, \"edges\": [",
+                D, cost, t.status, ok, gst_ref::tree_cost(t, input_graph),
+                gst_ref::mst_postprocess(t, input_graph).cost);
     for (size_t i = 0; i < t.edges.size(); ++i)
       std::printf("%s[%d, %d]", i ? ", " : "", t.edges[i].first, t.edges[i].second);
     std::printf("]}\n");

@@ -35,14 +35,30 @@ def test_emulated_kernels_match_oracle():
     assert out["fails"] == [], out["fails"]
 
 
-def test_reference_shaped_adapters(tmp_path):
-    _need_emu()
+def _adapter_demo(tmp_path):
     exe = tmp_path / "adapter_demo"
     cc = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
                          os.path.join(ROOT, "tests", "cpp", "adapter_demo.cpp"),
                          EMU, f"-Wl,-rpath,{os.path.dirname(EMU)}", "-o", str(exe)],
                         capture_output=True, text=True)
     assert cc.returncode == 0, cc.stderr
+    return exe
+
+
+def test_adapter_feasibility_and_mst_selftest(tmp_path):
+    _need_emu()
+    exe = _adapter_demo(tmp_path)
+    r = subprocess.run([str(exe), "selftest"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    got = dict(line.split()[1:] for line in r.stdout.splitlines())
+    assert got == {"path_feasible": "1", "path_cost": "7", "path_mst_cost": "3",
+                   "split_feasible": "0", "split_why": "disconnected",
+                   "part_feasible": "0", "part_why": "uncovered", "single_feasible": "1"}
+
+
+def test_reference_shaped_adapters(tmp_path):
+    _need_emu()
+    exe = _adapter_demo(tmp_path)
     g, grp = workloads.small(24, 50, 5, seed_=77, whi=9, member_p=0.2)
     u, v, w = g.src, g.dst, g.w
     lines = [f"{g.n} {len(u)}"] + [f"{a} {b} {c}" for a, b, c in zip(u, v, w)]
@@ -67,6 +83,11 @@ def test_reference_shaped_adapters(tmp_path):
             assert res["status"] == 1 and res["cost"] == (0 if D < 0 else 100000)
             continue
         assert res["status"] == 0 and res["cost"] == ref["cost"]
+        # the reference's feasibility check passes; the tree costs what was reported and
+        # MST post-processing cannot improve an optimal plain tree
+        assert res["feasible"] == 1 and res["tree_cost"] == res["cost"]
+        if D < 0:
+            assert res["mst_cost"] == res["cost"]
         if res["edges"]:
             e = np.array(res["edges"], np.int32)
             code, wsum = O.verify_tree(g, grp, qa, e, D)
