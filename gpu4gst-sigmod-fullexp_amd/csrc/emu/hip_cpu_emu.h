@@ -328,6 +328,12 @@ template <class T>
 inline T __hip_atomic_load(const T* p, int, int) {
   return *p;
 }
+template <class T>
+inline T __hip_atomic_fetch_min(T* p, T v, int, int) {
+  T o = *p;
+  if (v < o) *p = v;
+  return o;
+}
 
 // ---- runtime API
 inline const char* hipGetErrorString(hipError_t e) { return e ? "emulated error" : "no error"; }

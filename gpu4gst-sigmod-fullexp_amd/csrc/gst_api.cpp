@@ -155,7 +155,7 @@ constexpr double kWorklistFraction = 0.04;
 // Table footprint of one slot (int32 elements) and its worklist share.
 int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
   const int64_t rows = rows_of(k, D);
-  return 2 * rows * g->Vp + std::max<int64_t>(rows, 8) * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
+  return 2 * rows * g->Vp + rows * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
 }
 
 int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
@@ -299,8 +299,9 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     d.D = q.D;
     d.nsub = 1 << q.k;
     d.vv_off = s * VV;
+    d.nrows = (int32_t)rows;
     t_tot += rows * g->Vp;
-    e_tot += std::max<int64_t>(rows, 8) * g->Vp;
+    e_tot += rows * g->Vp;
     items += slot_items(g, q.k, q.D);
     sssp_items += g->nch();  // one-label items are (slot, vertex[, chunk])
     sd[s] = d;
@@ -362,7 +363,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(hipEventRecord(g->ph[0], g->stream));
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
   launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
-  launch_init_slot_tables(c, nslots, 2 * t_tot, (int64_t)nslots * g->Vp * CROW, g->stream);
+  launch_init_slot_tables(c, nslots, (int64_t)nslots * g->Vp * CROW, g->stream);
   const Task* dt = g->tasks.as<Task>();
   const int nt = (int)tasks.size();
   uint32_t* cnt = g->cnt.as<uint32_t>();

@@ -18,6 +18,7 @@ constexpr int BLOCK = 256;          // threads per workgroup of the expand kerne
 constexpr int WAVE = 64;            // gfx950 wavefront
 constexpr int RSTRIDE = 72;         // ints per subset in the route tables: 8x8 route + 8 min-route
 constexpr int RTABLE = 128 * RSTRIDE;  // route ints per query slot (2^7 subsets)
+constexpr int CSTAMP = 7;              // word of a distance row holding the one-label stamp
 constexpr int CROW = 8;             // one-label distances per vertex (k <= 7, padded to 8)
 constexpr int VV = 64;              // 8x8 group-to-group distances per slot
 
@@ -51,15 +52,26 @@ __host__ __device__ inline uint32_t wit_onelabel(uint32_t v, uint32_t p) {
 
 // Per-query-slot layout; all offsets count int32 elements of their arena.
 struct SlotDesc {
-  int64_t t_off;   // state words: entry (d * nsub + p) * Vp + v holds {cost, completion bound}
-                   // as two int32; the cost word of row 0 holds the group mask of v
-  int64_t e_off;   // enqueue stamps, same indexing as T (rows 0..k-1 serve the one-label phase)
+  int64_t t_off;   // state words, vertex-major: entry v * nrows + d * nsub + p holds
+                   // {cost, completion bound} as two int32, so all states of one vertex
+                   // (every merge partner) share a few cache lines; the cost word of
+                   // row 0 holds the group mask of v
+  int64_t e_off;   // enqueue stamps of DP states (iteration numbers), same indexing as T
   int64_t c_off;   // one-label distances c[v * 8 + i]
   int64_t r_off;   // route tables [p][72]
   int32_t k, full, D, nsub;  // D < 0 = plain problem
   int32_t vv_off;  // group distance matrix [8][8] in the vv array
-  int32_t pad[3];
+  int32_t nrows;   // rows per vertex: nsub (plain) or nsub * (D + 1) (hop)
+  int32_t pad[2];
 };
+
+// Index (in int2 entries of T, or int32 entries of E) of row `row` of vertex v.
+__host__ __device__ inline int64_t t_ix(const SlotDesc& S, int64_t row, int64_t v) {
+  return S.t_off + v * S.nrows + row;
+}
+__host__ __device__ inline int64_t e_ix(const SlotDesc& S, int64_t row, int64_t v) {
+  return S.e_off + v * S.nrows + row;
+}
 
 // Counters accumulated on device (uint64 each).
 enum Ctr {
@@ -145,8 +157,7 @@ void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_u
                       int32_t* out_eid, int32_t* out_n, int32_t* out_err, hipStream_t st);
 void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
                      float* gw, int64_t nund, hipStream_t st);
-void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
-                             hipStream_t st);
+void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st);
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
 void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st);
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);

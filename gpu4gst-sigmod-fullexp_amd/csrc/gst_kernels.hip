@@ -190,6 +190,34 @@ __device__ inline int find_item(const int32_t* s_off, int32_t e) {
   return lo;
 }
 
+// One-label stamps: word 7 of the distance row of (slot, v) (k <= 7 labels
+// use words 0..6) is (iteration << 8) | mask, mask = the labels of v that
+// changed since v was last queued; it shares the cache line the relaxation
+// just read. Rows are re-initialised (stamp 0) for every batch. Marks the
+// labels `bits` of v for iteration `it`, starting from the caller's copy
+// `old` of the word; returns true when this call queued v for `it` (the
+// caller appends it). No atomic is issued when the word already holds `it`
+// and the bits.
+__device__ inline bool mark_labels(int32_t* word, uint32_t old, uint32_t it, uint32_t bits) {
+  uint32_t* w = reinterpret_cast<uint32_t*>(word);
+  while (true) {
+    const bool queued = (old >> 8) == it;
+    if (queued && (old & bits) == bits) return false;
+    const uint32_t nw = queued ? (old | bits) : ((it << 8) | bits);
+    const uint32_t seen = atomicCAS(w, old, nw);
+    if (seen == old) return !queued;
+    old = seen;
+  }
+}
+
+// Labels an item of iteration `it` relaxes, from the stamp word w: the marked
+// ones, or all of them when v was marked again for a later iteration before
+// this item ran (the marks of `it` are then merged away, so nothing may be
+// skipped).
+__device__ inline uint32_t pending_labels(uint32_t w, uint32_t it) {
+  return (w >> 8) == it ? (w & 0xffu) : 0xffu;
+}
+
 __device__ inline void flush(unsigned long long* ctr, int idx, unsigned long long v) {
   // wave-reduce, then one atomic per wave
 #pragma unroll
@@ -218,7 +246,7 @@ __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t val) {
 __global__ void k_zero_row0(DevCtx c) {
   const SlotDesc S = c.slots[blockIdx.y];
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.Vp; v += gridDim.x * blockDim.x)
-    c.T[2 * (S.t_off + v)] = 0;
+    c.T[2 * t_ix(S, 0, v)] = 0;
 }
 
 void launch_fill_i32(int32_t* p, int64_t n, int32_t val, hipStream_t st) {
@@ -235,10 +263,21 @@ void launch_fill_u64(uint64_t* p, int64_t n, uint64_t val, hipStream_t st) {
   hipLaunchKernelGGL(k_fill_u64, (int)blocks, 256, 0, st, p, n, val);
 }
 
-void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t t_total, int64_t c_total,
-                             hipStream_t st) {
-  launch_fill_i32(c.T, t_total, INF, st);
-  launch_fill_i32(c.C, c_total, INF, st);
+// The state words of rows p >= 1 are written whole by k_lb_table ({INF,
+// bound}) before the DP reads them, so only row 0 (group masks) is cleared here.
+__global__ void k_init_crows(int32_t* C, int64_t rows) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
+       r += (int64_t)gridDim.x * blockDim.x) {
+    int4* p = reinterpret_cast<int4*>(C + r * CROW);
+    p[0] = make_int4(INF, INF, INF, INF);
+    p[1] = make_int4(INF, INF, INF, 0);  // word 7: one-label stamp
+  }
+}
+
+void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st) {
+  const int64_t rows = c_total / CROW;
+  const int64_t blocks = (rows + 255) / 256;
+  hipLaunchKernelGGL(k_init_crows, (int)(blocks < 4096 ? blocks : 4096), 256, 0, st, c.C, rows);
   launch_fill_i32(c.vv, (int64_t)nslots * VV, INF, st);
   launch_fill_u64(c.bestw, nslots, ((uint64_t)INF << 32) | 0xFFFFFFFFull, st);
   dim3 grid((c.Vp + 255) / 256 < 1024 ? (c.Vp + 255) / 256 : 1024, nslots);
@@ -262,9 +301,12 @@ __global__ __launch_bounds__(BLOCK) void k_groups_setup(DevCtx c, const Task* ta
       uint32_t v = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        atomicOr(&c.T[2 * (S.t_off + v)], 1 << tk.bit);
+        atomicOr(&c.T[2 * t_ix(S, 0, v)], 1 << tk.bit);
         atomicMin(&c.C[S.c_off + (int64_t)v * CROW + tk.bit], 0);
-        if (atomicMax(&c.E[S.e_off + v], stamp) < stamp) want = nchunks(c, v);
+        int32_t* sw = &c.C[S.c_off + (int64_t)v * CROW + CSTAMP];
+        const uint32_t old = __hip_atomic_load(reinterpret_cast<uint32_t*>(sw), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        if (mark_labels(sw, old, (uint32_t)stamp, 1u << tk.bit)) want = nchunks(c, v);
       }
       wave_append(want, mk_item(v, 0, 0, 0, (uint32_t)tk.slot), wl, cnt, c.cap, c.ctr);
     }
@@ -291,10 +333,9 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_beg[BLOCK];
   __shared__ int4 s_cv[BLOCK][2];      // the item's distances (masked to INF above the radius)
   __shared__ int64_t s_cbase[BLOCK];   // C offset of the item's slot
-  __shared__ int64_t s_ebase[BLOCK];   // stamp row of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | k << 1 | plain
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
-  const int32_t next = *c.iter + 1;
+  const int32_t cur = *c.iter, next = cur + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
   unsigned long long n_items = 0, n_relax = 0, n_upd = 0, n_noub = 0;
@@ -311,11 +352,13 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       const int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
       int4 a = *reinterpret_cast<const int4*>(row);
       int4 b = *reinterpret_cast<const int4*>(row + 4);
+      const uint32_t pend = pending_labels((uint32_t)b.w, (uint32_t)cur);
       int32_t cv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
       int32_t lo = INF;
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
-        if (i >= S.k || cv[i] >= radius) cv[i] = INF;
+        // unchanged labels were relaxed with these same values already
+        if (i >= S.k || cv[i] >= radius || !((pend >> i) & 1u)) cv[i] = INF;
         lo = min(lo, cv[i]);
       }
       if (lo < INF) {
@@ -328,7 +371,6 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         s_cv[tid][0] = make_int4(cv[0], cv[1], cv[2], cv[3]);
         s_cv[tid][1] = make_int4(cv[4], cv[5], cv[6], cv[7]);
         s_cbase[tid] = S.c_off;
-        s_ebase[tid] = S.e_off;
         s_meta[tid] = (s << 8) | ((uint32_t)S.k << 1) | (S.D < 0 ? 1u : 0u);
         ++n_items;
       }
@@ -352,15 +394,19 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         const int4 b = *reinterpret_cast<const int4*>(row + 4);
         int32_t cu[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
         ++n_relax;
-        bool improved = false;
+        // The atomics return nothing, so the wave does not wait on them.
+        // "Improved" is judged against the row just read: when another lane
+        // lowered the label in between, u is queued once more than needed,
+        // and cand, a real path length, still gives a valid upper bound.
+        uint32_t improved = 0;
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
           const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
           if (cv[i] < INF && cand < cu[i]) {
-            const int32_t old = atomicMin(&row[i], cand);
+            __hip_atomic_fetch_min(&row[i], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             ++n_upd;
-            if (cand < old) improved = true;
-            cu[i] = min(cand, old);
+            improved |= 1u << i;
+            cu[i] = cand;
           }
         }
         if (improved) {
@@ -376,7 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
               }
             if (fin && sum < INF) ub_update(&c.bestw[sl], (int32_t)sum, wit_onelabel(u, 0));
           }
-          if (atomicMax(&c.E[s_ebase[j] + u], next) < next) {
+          if (mark_labels(&row[CSTAMP], (uint32_t)cu[CSTAMP], (uint32_t)next, improved)) {
             cnt = nchunks(c, u);
             item = mk_item(u, 0, 0, 0, sl);
           }
@@ -553,36 +599,49 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Completion bounds: the second word of every state (v, p[, d]) holds
+// Completion bounds, written together with the initial cost INF as one 8-byte
+// store per state: the second word of every state (v, p[, d]) holds
 // lb(v, full ^ p), the lower bound of any tree containing v that touches the
 // groups outside p (the reference's lb0 table, DPQ.cu:193-243, evaluated here
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
-  __shared__ int32_t s_r[RSTRIDE];
-  const int s = blockIdx.z;
+  // The slot's route tables, all subsets, re-strided to an odd row length:
+  // lanes of a wave read the rows of consecutive subsets, and with 73 words
+  // per row those rows start in distinct LDS banks (72 would put every 8th
+  // lane on the same bank).
+  constexpr int LSTRIDE = RSTRIDE + 1;
+  __shared__ int32_t s_r[128 * LSTRIDE];
+  const int s = blockIdx.y;
   const SlotDesc S = c.slots[s];
-  const uint32_t p = blockIdx.y + 1;
-  if ((int32_t)p >= S.nsub) return;
-  const uint32_t U = (uint32_t)S.full ^ p;
-  for (int x = threadIdx.x; x < RSTRIDE; x += blockDim.x) s_r[x] = c.R[S.r_off + U * RSTRIDE + x];
+  for (int x = threadIdx.x; x < S.nsub * RSTRIDE; x += blockDim.x)
+    s_r[(x / RSTRIDE) * LSTRIDE + x % RSTRIDE] = c.R[S.r_off + x];
   __syncthreads();
   const int layers = S.D < 0 ? 1 : S.D + 1;
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.V; v += gridDim.x * blockDim.x) {
+  // one thread per (v, p), p fastest: the threads of a wave write consecutive
+  // entries of the vertex-major table and share the vertex's distance row
+  const int64_t total = (int64_t)c.V << S.k;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = (uint32_t)(i & (S.nsub - 1));
+    if (p == 0) continue;
+    const int64_t v = i >> S.k;
+    const uint32_t U = (uint32_t)S.full ^ p;
     int32_t lb = 0;
     if (U) {
       int32_t sumc;
-      lb = lb_eval(&c.C[S.c_off + (int64_t)v * CROW], s_r, U, INF, INF, &sumc);
+      lb = lb_eval(&c.C[S.c_off + v * CROW], s_r + U * LSTRIDE, U, INF, INF, &sumc);
     }
     for (int d = 0; d < layers; ++d)
-      c.T[2 * (S.t_off + ((int64_t)d * S.nsub + p) * c.Vp + v) + 1] = lb;
+      *reinterpret_cast<int2*>(&c.T[2 * t_ix(S, (int64_t)d * S.nsub + p, v)]) = make_int2(INF, lb);
   }
 }
 
 void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) {
-  if (nslots <= 0 || max_nsub <= 1) return;
-  const int bx = (c.V + 255) / 256 < 64 ? (c.V + 255) / 256 : 64;
-  dim3 grid(bx, max_nsub - 1, nslots);
+  if (nslots <= 0) return;  // must run for every batch: it initialises the state words
+  const int64_t work = (int64_t)c.V * max_nsub;
+  const int bx = (int)((work + 255) / 256 < 128 ? (work + 255) / 256 : 128);
+  dim3 grid(bx, nslots);
   hipLaunchKernelGGL(k_lb_table, grid, 256, 0, st, c);
 }
 
@@ -604,7 +663,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
       uint32_t v = 0, gsv = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        gsv = (uint32_t)c.T[2 * (S.t_off + v)];
+        gsv = (uint32_t)c.T[2 * t_ix(S, 0, v)];
         if ((uint32_t)__builtin_ctz(gsv | 0x100u) != (uint32_t)tk.bit) gsv = 0;
       }
       uint32_t p = gsv;  // enumerate non-empty submasks of gsv
@@ -613,7 +672,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
         uint32_t want = 0;
         uint64_t item = 0;
         if (live) {
-          atomicMin(&c.T[2 * (S.t_off + (int64_t)p * c.Vp + v)], 0);
+          atomicMin(&c.T[2 * t_ix(S, p, v)], 0);
           if ((int32_t)p == S.full) {
             ub_update(&c.bestw[tk.slot], 0, wit_full(v, 0));
           } else {
@@ -622,7 +681,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
                                          c.sched[tk.slot].R);
               if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
             }
-            if (atomicMax(&c.E[S.e_off + (int64_t)p * c.Vp + v], stamp) < stamp) {
+            if (atomicMax(&c.E[e_ix(S, p, v)], stamp) < stamp) {
               want = nchunks(c, v);
               item = mk_item(v, p, 0, 0, (uint32_t)tk.slot);
             }
@@ -667,13 +726,13 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
   __shared__ int32_t s_cost[BLOCK];
-  __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row
+  __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row of vertex 0
+  __shared__ int32_t s_rows[BLOCK];    // rows per vertex of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
-  const int64_t Vp = c.Vp;
   unsigned long long nF = 0, nRg = 0, nRm = 0, nU = 0, nLb = 0, nItems = 0;
 
   for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
@@ -697,7 +756,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       S = c.slots[s];
       Dq = S.D;
       const int64_t row = HOP ? (int64_t)d * S.nsub + p : (int64_t)p;
-      const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * (S.t_off + row * Vp + v)]);
+      const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, row, v)]);
       t = tl.x;
       Sched* sc = &c.sched[s];
       bool expand = t < INF;
@@ -709,10 +768,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         const bool half = t < best / 2 + 2;  // 1/2 rule: costlier states are never expanded
         if (ch == 0 && (!half || t < sc->thr)) {
           const uint32_t xs = (uint32_t)S.full ^ p;
-          const int32_t t2 = c.T[2 * (S.t_off + (int64_t)xs * Vp + v)];
+          const int32_t t2 = c.T[2 * t_ix(S, xs, v)];
           if (t2 < INF && t < INF) {
             const int32_t mcost = t + t2;
-            int32_t* dfull = &c.T[2 * (S.t_off + (int64_t)S.full * Vp + v)];
+            int32_t* dfull = &c.T[2 * t_ix(S, S.full, v)];
             if (mcost < *dfull && mcost < atomicMin(dfull, mcost))
               ub_update(&c.bestw[s], mcost, wit_full(v, 0));
           }
@@ -722,7 +781,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       if (expand && t >= sc->thr) {  // not in the current cost band yet
         expand = false;
         if (t < sc->mindef) atomicMin(&sc->mindef, t);
-        if (atomicMax(&c.E[S.e_off + row * Vp + v], next) < next) {
+        if (atomicMax(&c.E[e_ix(S, row, v)], next) < next) {
           carry_cnt = nchunks(c, v);
           citem = mk_item(v, p, d, 0, s);
         }
@@ -743,7 +802,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         deg = (HOP && (int32_t)d >= Dq) ? 0 : max(0, re - rb);
         s_beg[tid] = rb;
         s_cost[tid] = t;
-        s_dst[tid] = S.t_off + (HOP ? ((int64_t)(d + 1) * S.nsub + p) : (int64_t)p) * Vp;
+        s_dst[tid] = S.t_off + (HOP ? ((int64_t)(d + 1) * S.nsub + p) : (int64_t)p);
+        s_rows[tid] = S.nrows;
         s_meta[tid] = (s << 12) | (d << 8) | p;
         if (ch == 0) mask = (uint32_t)S.full ^ p;
       }
@@ -785,14 +845,14 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           for (int i = 0; i < 8; ++i)
             if ((p2 >> i) & 1u) lbp = max(lbp, cvm[i]);
           const int64_t src = HOP ? ((int64_t)d2 * S.nsub + p2) : (int64_t)p2;
-          const int32_t t2 = lbp > mlim ? INF : c.T[2 * (S.t_off + src * Vp + v)];
+          const int32_t t2 = lbp > mlim ? INF : c.T[2 * t_ix(S, src, v)];
           nRm += lbp <= mlim;
           if (t2 < INF && t2 <= mlim) {
             const int32_t mc = t + t2;
             const uint32_t P = p | p2;
             const uint32_t nd = HOP ? max(d, d2) : 0u;
             const int64_t drow = HOP ? ((int64_t)nd * S.nsub + P) : (int64_t)P;
-            int32_t* dst = &c.T[2 * (S.t_off + drow * Vp + v)];
+            int32_t* dst = &c.T[2 * t_ix(S, drow, v)];
             const int2 dl = *reinterpret_cast<const int2*>(dst);
             if (mc < dl.x) {
               if ((int32_t)P == S.full) {
@@ -823,7 +883,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                   }
                   // 2/3 rule (Conditional Tree Merging) for the queue only
                   const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
-                  if (enq && atomicMax(&c.E[S.e_off + drow * Vp + v], next) < next) {
+                  if (enq && atomicMax(&c.E[e_ix(S, drow, v)], next) < next) {
                     want = nchunks(c, v);
                     item = mk_item(v, P, nd, 0, s);
                   }
@@ -858,7 +918,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         const uint32_t u = (uint32_t)c.col[ej];
         const int32_t gc = s_cost[j] + c.wt[ej];
         const int64_t dbase = s_dst[j];
-        int32_t* dst = &c.T[2 * (dbase + u)];
+        const int64_t uoff = (int64_t)u * s_rows[j];
+        int32_t* dst = &c.T[2 * (dbase + uoff)];
         const int2 dl = *reinterpret_cast<const int2*>(dst);
         ++nRg;
         if (gc < dl.x) {
@@ -881,7 +942,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                 ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
             }
             const int64_t erow = dbase - Sj.t_off + Sj.e_off;
-            if (atomicMax(&c.E[erow + u], next) < next) {
+            if (atomicMax(&c.E[erow + uoff], next) < next) {
               want = nchunks(c, u);
               item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
             }
@@ -938,7 +999,6 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
   const int s = blockIdx.x;
   const int lane = threadIdx.x;
   const SlotDesc S = c.slots[s];
-  const int64_t Vp = c.Vp;
   const uint64_t bw = c.bestw[s];
   const int32_t cost = (int32_t)(bw >> 32);
   const uint32_t wit = (uint32_t)bw;
@@ -959,7 +1019,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     return;
   }
   auto Tat = [&](uint32_t v, uint32_t p, uint32_t d) -> int32_t {
-    return c.T[2 * (S.t_off + ((int64_t)d * S.nsub + p) * Vp + v)];
+    return c.T[2 * t_ix(S, (int64_t)d * S.nsub + p, v)];
   };
   auto emit = [&](uint32_t a, uint32_t b2, int32_t ej) {  // lane 0 only
     if (s_ne >= max_edges) {
@@ -1035,7 +1095,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     const int32_t b = st_b[top];
     __syncthreads();
     if (lane == 0) s_sp = top;
-    const uint32_t gs = (uint32_t)c.T[2 * (S.t_off + v)];
+    const uint32_t gs = (uint32_t)c.T[2 * t_ix(S, 0, v)];
     __syncthreads();
     if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
     // merge splits: proper non-empty submasks p1 (hop: with layers d1, d2,

@@ -61,6 +61,13 @@ def main(n_plain, n_hop):
                     fails.append(("batch", per_pass, seed, q.tolist(), int(b.costs[i]),
                                   int(ref["cost"])))
         G.set_option("sssp_slots", 0)
+        # single queries again after the batch: the slot layout changes between
+        # calls, so stamps left by the batch sit where these queries keep theirs
+        for q in reversed(qs):
+            r = G.solve(q)
+            ref = O.dpbf(g, grp, q)
+            if ref["cost"] < O.INF() and r.cost != ref["cost"]:
+                fails.append(("after-batch", seed, q.tolist(), int(r.cost), int(ref["cost"])))
         G.close()
     for seed in range(n_hop):
         rng = np.random.default_rng(6000 + seed)
@@ -82,6 +89,22 @@ def main(n_plain, n_hop):
             if not ok:
                 fails.append(("hop", seed, D, q.tolist(), int(r.cost), int(ref["cost"])))
         G.close()
+    # calls of different shapes on one graph: a batch of mixed k, then the same
+    # queries one at a time (each call lays out its slots differently)
+    g, grp = workloads.small(150, 400, 10, seed_=17, whi=60, member_p=0.04)
+    G = GSTGraph.from_generated(g)
+    G.set_groups(grp)
+    rng = np.random.default_rng(21)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 24)]
+    refs = [O.dpbf(g, grp, q)["cost"] for q in qs]
+    br = G.solve_batch(qs)
+    for i in reversed(range(len(qs))):
+        r = G.solve(qs[i])
+        cases += 1
+        for got, what in ((int(br.costs[i]), "mixed-batch"), (int(r.cost), "mixed-single")):
+            if refs[i] < O.INF() and got != refs[i]:
+                fails.append((what, i, qs[i].tolist(), got, int(refs[i])))
+    G.close()
     print(json.dumps({"cases": cases, "fails": fails}))
 
 

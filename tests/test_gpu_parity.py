@@ -110,6 +110,34 @@ def test_batch_equals_single(gpu):
         _check_tree(g, grp, q, bh.costs[i], bh.tree(i), int(Ds[i]))
 
 
+def test_calls_of_different_shapes_on_one_graph(gpu):
+    # Batches and single queries of different k and D reuse the same device
+    # buffers with different slot layouts: stamps and labels left behind by
+    # one call must never change the next call's answer.
+    g, grp = workloads.small(150, 400, 10, seed_=17, whi=60, member_p=0.04)
+    G = _graph(g, grp)
+    rng = np.random.default_rng(21)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 24)]
+    refs = [O.dpbf(g, grp, q)["cost"] for q in qs]
+    for rnd in range(2):
+        _say(f"mixed calls round {rnd}: batch of 24, then singles, then hop singles")
+        br = G.solve_batch(qs)
+        for i in range(len(qs)):
+            if refs[i] < O.INF():
+                assert br.costs[i] == refs[i], ("batch", rnd, i)
+        for i in reversed(range(len(qs))):
+            r = G.solve(qs[i])
+            if refs[i] < O.INF():
+                assert r.cost == refs[i], ("single", rnd, i, r.cost, refs[i])
+        for i in range(0, len(qs), 6):
+            D = 1 + i % 4
+            r = G.solve(qs[i], D=D)
+            ref = O.hop(g, grp, qs[i], D)
+            assert r.status == (STATUS_OK if ref["status"] == O.OK else STATUS_INFEASIBLE)
+            if ref["status"] == O.OK:
+                assert r.cost == ref["cost"], ("hop", rnd, i, D)
+
+
 def test_infeasible_and_trivial(gpu):
     # two components: {0,1,2} and {3,4}
     g = gen.from_edges(5, [0, 1, 3], [1, 2, 4], [4, 5, 6])
