@@ -254,6 +254,9 @@ void launch(dim3 grid, dim3 block, F kernel, A... args) {
 
 inline void __syncthreads() { emu::wait(emu::SYNC, 0); }
 inline int __lane_id() { return (int)(threadIdx.x % emu::WAVE); }
+// callers pass wave-uniform values only
+template <class T>
+inline T __builtin_amdgcn_readfirstlane(T x) { return x; }
 inline uint64_t __ballot(int p) { return emu::wait(emu::BALLOT, p != 0); }
 inline int __any(int p) { return (int)emu::wait(emu::ANY, p != 0); }
 

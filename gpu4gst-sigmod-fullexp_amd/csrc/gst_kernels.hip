@@ -32,6 +32,51 @@ namespace {
 
 __device__ inline int lane_id() { return __lane_id(); }
 
+#ifndef GST_EPL
+#define GST_EPL 4
+#endif
+constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
+
+// Appends, for each lane, EPL entries of `cnt[q]` chunk items
+// (items[q] | chunk << 36) with ONE global atomicAdd per workgroup: waves
+// reserve their share of the workgroup's range with an LDS atomic, one
+// thread reserves the range in the list. Every thread of the workgroup must
+// call this together (it synchronises twice); `round` alternates the LDS
+// accumulator so a fast wave's next call cannot meet a stale one.
+__device__ inline void block_append(const uint32_t* cnt, const uint64_t* items, uint64_t* out,
+                                    uint32_t* out_cnt, uint32_t cap, unsigned long long* ctr,
+                                    uint32_t* s_acc, uint32_t* s_base, int round) {
+  const int lane = lane_id();
+  uint32_t mine = 0;
+#pragma unroll
+  for (int q = 0; q < EPL; ++q) mine += cnt[q];
+  uint32_t incl = mine;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, WAVE);
+    if (lane >= off) incl += y;
+  }
+  const uint32_t wave_total = __shfl(incl, WAVE - 1, WAVE);
+  uint32_t* acc = &s_acc[round & 1];
+  uint32_t wave_off = 0;
+  if (lane == 0 && wave_total) wave_off = atomicAdd(acc, wave_total);
+  wave_off = __shfl(wave_off, 0, WAVE);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint32_t n = *acc;
+    *s_base = n ? atomicAdd(out_cnt, n) : 0u;
+    s_acc[(round + 1) & 1] = 0;
+  }
+  __syncthreads();
+  uint32_t pos = *s_base + wave_off + (incl - mine);
+#pragma unroll
+  for (int q = 0; q < EPL; ++q)
+    for (uint32_t i = 0; i < cnt[q]; ++i, ++pos) {
+      if (pos < cap) out[pos] = items[q] | ((uint64_t)i << 36);
+      else atomicAdd(&ctr[C_OVERFLOW], 1ull);
+    }
+}
+
 // Appends `cnt` chunk items (item | chunk << 36) per lane; every lane of the
 // wavefront must call this together.
 __device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, uint32_t* out_cnt,
@@ -334,6 +379,9 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int4 s_cv[BLOCK][2];      // the item's distances (masked to INF above the radius)
   __shared__ int64_t s_cbase[BLOCK];   // C offset of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | k << 1 | plain
+  __shared__ uint32_t s_acc[2], s_abase;  // block_append
+  if (threadIdx.x < 2) s_acc[threadIdx.x] = 0;
+  int round = 0;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t cur = *c.iter, next = cur + 1;
   const int tid = threadIdx.x;
@@ -377,58 +425,93 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
     }
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
-    for (int32_t base = 0; base < total; base += BLOCK) {
-      const int32_t e = base + tid;
-      uint32_t cnt = 0;
-      uint64_t item = 0;
-      if (e < total) {
-        const int j = find_item(s_off, e);
-        const int32_t ej = s_beg[j] + (e - s_off[j]);
-        const uint32_t u = (uint32_t)c.col[ej];
-        const int32_t wgt = c.wt[ej];
-        const uint32_t meta = s_meta[j], sl = meta >> 8, k = (meta >> 1) & 7u;
-        const int4 va = s_cv[j][0], vb = s_cv[j][1];
-        const int32_t cv[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
-        int32_t* row = &c.C[s_cbase[j] + (int64_t)u * CROW];
-        const int4 a = *reinterpret_cast<const int4*>(row);
-        const int4 b = *reinterpret_cast<const int4*>(row + 4);
-        int32_t cu[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        ++n_relax;
-        // The atomics return nothing, so the wave does not wait on them.
-        // "Improved" is judged against the row just read: when another lane
-        // lowered the label in between, u is queued once more than needed,
-        // and cand, a real path length, still gives a valid upper bound.
-        uint32_t improved = 0;
+    // Each lane takes EPL edges per round (strided by the block size, so
+    // the CSR reads stay coalesced) and issues all their loads before using
+    // any: the random neighbour-row reads of one lane are then in flight
+    // together instead of one after another.
+    for (int32_t base = 0; base < total; base += BLOCK * EPL) {
+      int jq[EPL];
+      uint32_t uq[EPL];
+      int32_t wq[EPL];
+      int4 aq[EPL], bq[EPL];
 #pragma unroll
-        for (int i = 0; i < 8; ++i) {
-          const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
-          if (cv[i] < INF && cand < cu[i]) {
-            __hip_atomic_fetch_min(&row[i], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ++n_upd;
-            improved |= 1u << i;
-            cu[i] = cand;
-          }
-        }
-        if (improved) {
-          if (meta & 1u) {
-            // every finite distance is a real path length: u joined to all groups
-            int64_t sum = 0;
-            bool fin = true;
+      for (int q = 0; q < EPL; ++q) {
+        const int32_t e = base + q * BLOCK + tid;
+        jq[q] = e < total ? find_item(s_off, e) : -1;
+      }
 #pragma unroll
-            for (int i = 0; i < 8; ++i)
-              if (i < (int)k) {
-                fin = fin && cu[i] < INF;
-                sum += cu[i];
-              }
-            if (fin && sum < INF) ub_update(&c.bestw[sl], (int32_t)sum, wit_onelabel(u, 0));
-          }
-          if (mark_labels(&row[CSTAMP], (uint32_t)cu[CSTAMP], (uint32_t)next, improved)) {
-            cnt = nchunks(c, u);
-            item = mk_item(u, 0, 0, 0, sl);
-          }
+      for (int q = 0; q < EPL; ++q) {
+        uq[q] = 0;
+        wq[q] = 0;
+        if (jq[q] >= 0) {
+          const int32_t ej = s_beg[jq[q]] + (base + q * BLOCK + tid - s_off[jq[q]]);
+          uq[q] = (uint32_t)c.col[ej];
+          wq[q] = c.wt[ej];
         }
       }
-      wave_append(cnt, item, w.out, w.out_cnt, c.cap, c.ctr);
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        if (jq[q] >= 0) {
+          const int32_t* row = &c.C[s_cbase[jq[q]] + (int64_t)uq[q] * CROW];
+          aq[q] = *reinterpret_cast<const int4*>(row);
+          bq[q] = *reinterpret_cast<const int4*>(row + 4);
+        }
+      }
+      uint32_t cntq[EPL];
+      uint64_t itemq[EPL];
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        uint32_t cnt = 0;
+        uint64_t item = 0;
+        if (jq[q] >= 0) {
+          const int j = jq[q];
+          const uint32_t u = uq[q];
+          const int32_t wgt = wq[q];
+          const uint32_t meta = s_meta[j], sl = meta >> 8, k = (meta >> 1) & 7u;
+          const int4 va = s_cv[j][0], vb = s_cv[j][1];
+          const int32_t cv[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+          int32_t* row = &c.C[s_cbase[j] + (int64_t)u * CROW];
+          const int4 a = aq[q], b = bq[q];
+          int32_t cu[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+          ++n_relax;
+          // The atomics return nothing, so the wave does not wait on them.
+          // "Improved" is judged against the row just read: when another lane
+          // lowered the label in between, u is queued once more than needed,
+          // and cand, a real path length, still gives a valid upper bound.
+          uint32_t improved = 0;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
+            if (cv[i] < INF && cand < cu[i]) {
+              __hip_atomic_fetch_min(&row[i], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              ++n_upd;
+              improved |= 1u << i;
+              cu[i] = cand;
+            }
+          }
+          if (improved) {
+            if (meta & 1u) {
+              // every finite distance is a real path length: u joined to all groups
+              int64_t sum = 0;
+              bool fin = true;
+#pragma unroll
+              for (int i = 0; i < 8; ++i)
+                if (i < (int)k) {
+                  fin = fin && cu[i] < INF;
+                  sum += cu[i];
+                }
+              if (fin && sum < INF) ub_update(&c.bestw[sl], (int32_t)sum, wit_onelabel(u, 0));
+            }
+            if (mark_labels(&row[CSTAMP], (uint32_t)cu[CSTAMP], (uint32_t)next, improved)) {
+              cnt = nchunks(c, u);
+              item = mk_item(u, 0, 0, 0, sl);
+            }
+          }
+        }
+        cntq[q] = cnt;
+        itemq[q] = item;
+      }
+      block_append(cntq, itemq, w.out, w.out_cnt, c.cap, c.ctr, s_acc, &s_abase, ++round);
     }
     __syncthreads();
   }
@@ -606,43 +689,73 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
-  // The slot's route tables, all subsets, re-strided to an odd row length:
-  // lanes of a wave read the rows of consecutive subsets, and with 73 words
-  // per row those rows start in distinct LDS banks (72 would put every 8th
-  // lane on the same bank).
-  constexpr int LSTRIDE = RSTRIDE + 1;
-  __shared__ int32_t s_r[128 * LSTRIDE];
+  // A workgroup owns 64 consecutive vertices of one slot. Each wave takes
+  // every fourth subset p, so U = full ^ p is the same in all its lanes (one
+  // lane per vertex): the loops over the groups of U are scalar branches and
+  // the route table entries scalar loads. The bounds go to an LDS tile
+  // [vertex][p], which the workgroup then writes out as whole {INF, bound}
+  // state words, contiguous in the vertex-major table.
+  constexpr int TV = 64;
+  __shared__ int32_t tile[TV * 128];
   const int s = blockIdx.y;
   const SlotDesc S = c.slots[s];
-  for (int x = threadIdx.x; x < S.nsub * RSTRIDE; x += blockDim.x)
-    s_r[(x / RSTRIDE) * LSTRIDE + x % RSTRIDE] = c.R[S.r_off + x];
-  __syncthreads();
-  const int layers = S.D < 0 ? 1 : S.D + 1;
-  // one thread per (v, p), p fastest: the threads of a wave write consecutive
-  // entries of the vertex-major table and share the vertex's distance row
-  const int64_t total = (int64_t)c.V << S.k;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const uint32_t p = (uint32_t)(i & (S.nsub - 1));
-    if (p == 0) continue;
-    const int64_t v = i >> S.k;
-    const uint32_t U = (uint32_t)S.full ^ p;
+  const int64_t v0 = (int64_t)blockIdx.x * TV;
+  if (v0 >= c.V) return;
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int wv = threadIdx.x / WAVE;
+  const int64_t v = v0 + lane;
+  int32_t cs[8];
+  {
+    const int64_t vr = v < c.V ? v : v0;
+    const int4 a = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW]);
+    const int4 b = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW + 4]);
+    cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
+    cs[4] = b.x; cs[5] = b.y; cs[6] = b.z; cs[7] = b.w;
+  }
+  const int32_t* Rs = c.R + S.r_off;
+  for (int p = 1 + wv; p < S.nsub; p += BLOCK / WAVE) {
+    const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
     int32_t lb = 0;
     if (U) {
-      int32_t sumc;
-      lb = lb_eval(&c.C[S.c_off + v * CROW], s_r + U * LSTRIDE, U, INF, INF, &sumc);
+      const int32_t* R = Rs + U * RSTRIDE;
+      int32_t one = 0, mn = INF;
+      for (int i = 0; i < 8; ++i)
+        if ((U >> i) & 1u) {
+          one = max(one, cs[i]);
+          mn = min(mn, cs[i]);
+        }
+      if (one >= INF) {
+        lb = one;
+      } else {
+        int32_t lb1 = INF, lb2 = 0;
+        for (int i = 0; i < 8; ++i) {
+          if (!((U >> i) & 1u)) continue;
+          for (int j = 0; j < 8; ++j)
+            if ((U >> j) & 1u) lb1 = min(lb1, (cs[i] + R[i * 8 + j] + cs[j]) >> 1);
+          lb2 = max(lb2, (cs[i] + R[64 + i] + mn) >> 1);
+        }
+        lb = max(one, max(lb1, lb2));
+      }
     }
-    for (int d = 0; d < layers; ++d)
-      *reinterpret_cast<int2*>(&c.T[2 * t_ix(S, (int64_t)d * S.nsub + p, v)]) = make_int2(INF, lb);
+    tile[lane * 128 + p] = lb;
+  }
+  __syncthreads();
+  const int nv = (int)min<int64_t>(TV, c.V - v0);
+  const int nrows = S.nrows;
+  int2* T2 = reinterpret_cast<int2*>(c.T);
+  for (int x = threadIdx.x; x < nv * nrows; x += BLOCK) {
+    const int vl = x / nrows, row = x - vl * nrows;
+    if (row == 0) continue;  // row 0 holds the group mask
+    const int p = row & (S.nsub - 1);
+    T2[t_ix(S, row, v0 + vl)] = make_int2(INF, p ? tile[vl * 128 + p] : 0);
   }
 }
 
 void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) {
   if (nslots <= 0) return;  // must run for every batch: it initialises the state words
-  const int64_t work = (int64_t)c.V * max_nsub;
-  const int bx = (int)((work + 255) / 256 < 128 ? (work + 255) / 256 : 128);
-  dim3 grid(bx, nslots);
-  hipLaunchKernelGGL(k_lb_table, grid, 256, 0, st, c);
+  (void)max_nsub;
+  dim3 grid((c.V + 63) / 64, nslots);
+  hipLaunchKernelGGL(k_lb_table, grid, BLOCK, 0, st, c);
 }
 
 // ---------------------------------------------------------------------------
