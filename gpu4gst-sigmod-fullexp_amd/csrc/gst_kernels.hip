@@ -37,6 +37,89 @@ __device__ inline int lane_id() { return __lane_id(); }
 #endif
 constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
 
+// Appends `cnt` chunk items (item | chunk << 36) per lane; every lane of the
+// wavefront must call this together.
+__device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, uint32_t* out_cnt,
+                                   uint32_t cap, unsigned long long* ctr) {
+  const int lane = lane_id();
+  const uint64_t many = __ballot(cnt > 1);
+  uint32_t excl, total;
+  if (many == 0) {
+    const uint64_t m = __ballot(cnt == 1);
+    if (m == 0) return;
+    total = (uint32_t)__popcll(m);
+    excl = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+  } else {
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int off = 1; off < WAVE; off <<= 1) {
+      uint32_t y = __shfl_up(incl, off, WAVE);
+      if (lane >= off) incl += y;
+    }
+    total = __shfl(incl, WAVE - 1, WAVE);
+    excl = incl - cnt;
+  }
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(out_cnt, total);
+  base = __shfl(base, 0, WAVE);
+  for (uint32_t i = 0; i < cnt; ++i) {
+    uint32_t pos = base + excl + i;
+    if (pos < cap) out[pos] = item | ((uint64_t)i << 36);
+    else atomicAdd(&ctr[C_OVERFLOW], 1ull);
+  }
+}
+
+// Wave-private staging of worklist appends in LDS: items collect in the
+// wave's buffer and go to the list with one global atomicAdd per WSTAGE
+// items (the list counter is a single word every wave of the grid would
+// otherwise hit once per append round). `n` is the wave-uniform fill level.
+constexpr int WSTAGE = 256;
+
+__device__ inline void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ inline void stage_flush(uint64_t* stage, uint32_t& n, uint64_t* out,
+                                   uint32_t* out_cnt, uint32_t cap, unsigned long long* ctr) {
+  if (n == 0) return;
+  wave_sync();
+  uint32_t base = 0;
+  if (lane_id() == 0) base = atomicAdd(out_cnt, n);
+  base = __shfl(base, 0, WAVE);
+  for (uint32_t i = (uint32_t)lane_id(); i < n; i += WAVE) {
+    const uint32_t pos = base + i;
+    if (pos < cap) out[pos] = stage[i];
+    else atomicAdd(&ctr[C_OVERFLOW], 1ull);
+  }
+  wave_sync();
+  n = 0;
+}
+
+// Stages `cnt` chunk items (item | chunk << 36) per lane; every lane of the
+// wavefront must call this together.
+__device__ inline void stage_append(uint64_t* stage, uint32_t& n, uint32_t cnt, uint64_t item,
+                                    uint64_t* out, uint32_t* out_cnt, uint32_t cap,
+                                    unsigned long long* ctr) {
+  const int lane = lane_id();
+  uint32_t incl = cnt;
+#pragma unroll
+  for (int off = 1; off < WAVE; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, WAVE);
+    if (lane >= off) incl += y;
+  }
+  const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+  if (total == 0) return;
+  const uint32_t excl = incl - cnt;
+  if (total > (uint32_t)WSTAGE) {  // more than a buffer: straight to the list
+    wave_append(cnt, item, out, out_cnt, cap, ctr);
+    return;
+  }
+  if (n + total > (uint32_t)WSTAGE) stage_flush(stage, n, out, out_cnt, cap, ctr);
+  for (uint32_t i = 0; i < cnt; ++i) stage[n + excl + i] = item | ((uint64_t)i << 36);
+  n += total;
+}
+
 // Appends, for each lane, EPL entries of `cnt[q]` chunk items
 // (items[q] | chunk << 36) with ONE global atomicAdd per workgroup: waves
 // reserve their share of the workgroup's range with an LDS atomic, one
@@ -77,37 +160,6 @@ __device__ inline void block_append(const uint32_t* cnt, const uint64_t* items, 
     }
 }
 
-// Appends `cnt` chunk items (item | chunk << 36) per lane; every lane of the
-// wavefront must call this together.
-__device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, uint32_t* out_cnt,
-                                   uint32_t cap, unsigned long long* ctr) {
-  const int lane = lane_id();
-  const uint64_t many = __ballot(cnt > 1);
-  uint32_t excl, total;
-  if (many == 0) {
-    const uint64_t m = __ballot(cnt == 1);
-    if (m == 0) return;
-    total = (uint32_t)__popcll(m);
-    excl = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-  } else {
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-      uint32_t y = __shfl_up(incl, off, WAVE);
-      if (lane >= off) incl += y;
-    }
-    total = __shfl(incl, WAVE - 1, WAVE);
-    excl = incl - cnt;
-  }
-  uint32_t base = 0;
-  if (lane == 0) base = atomicAdd(out_cnt, total);
-  base = __shfl(base, 0, WAVE);
-  for (uint32_t i = 0; i < cnt; ++i) {
-    uint32_t pos = base + excl + i;
-    if (pos < cap) out[pos] = item | ((uint64_t)i << 36);
-    else atomicAdd(&ctr[C_OVERFLOW], 1ull);
-  }
-}
 
 __device__ inline void ub_update(uint64_t* bw, int32_t cost, uint32_t wit) {
   const uint64_t pk = ((uint64_t)(uint32_t)cost << 32) | wit;
@@ -842,6 +894,9 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row of vertex 0
   __shared__ int32_t s_rows[BLOCK];    // rows per vertex of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
+  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
+  uint64_t* stage = s_stage[threadIdx.x / WAVE];
+  uint32_t st_n = 0;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
@@ -922,7 +977,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       }
     }
 
-    wave_append(carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr);
+    stage_append(stage, st_n, carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr);
 
     // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
     {
@@ -1014,58 +1069,85 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           }
           live = p2 != 0;
         }
-        wave_append(want, item, w.out, w.out_cnt, c.cap, c.ctr);
+        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
       }
     }
 
     // ---- grow: the workgroup's edges, balanced by an LDS prefix scan
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
-    for (int32_t base = 0; base < total; base += BLOCK) {
-      const int32_t e = base + tid;
-      uint32_t want = 0;
-      uint64_t item = 0;
-      if (e < total) {
-        const int j = find_item(s_off, e);
-        const int32_t ej = s_beg[j] + (e - s_off[j]);
-        const uint32_t u = (uint32_t)c.col[ej];
-        const int32_t gc = s_cost[j] + c.wt[ej];
-        const int64_t dbase = s_dst[j];
-        const int64_t uoff = (int64_t)u * s_rows[j];
-        int32_t* dst = &c.T[2 * (dbase + uoff)];
-        const int2 dl = *reinterpret_cast<const int2*>(dst);
-        ++nRg;
-        if (gc < dl.x) {
-          const uint32_t meta = s_meta[j];
-          const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
-          int32_t best = INF;
-          bool keep = true;
-          if (!HOP || c.hop_prune) {  // completion bound before the atomic, as for merges
-            best = best_of(&c.bestw[sl]);
-            keep = dl.y < INF && (best >= INF || gc + dl.y <= (HOP ? best : best + 2));
-          }
-          if (keep && gc < atomicMin(dst, gc)) {
-            ++nU;
-            const SlotDesc Sj = c.slots[sl];
-            if (!HOP) {
-              ++nLb;
-              const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)u * CROW], (uint32_t)Sj.full ^ pd,
-                                         c.sched[sl].R);
-              if (sumc < INF && gc + sumc < best)
-                ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+    // EPL edges per lane per round, all loads issued before any is used
+    for (int32_t base = 0; base < total; base += BLOCK * EPL) {
+      int jq[EPL];
+      uint32_t uq[EPL];
+      int32_t gq[EPL];
+      int2 dq[EPL];
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        const int32_t e = base + q * BLOCK + tid;
+        jq[q] = e < total ? find_item(s_off, e) : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        uq[q] = 0;
+        gq[q] = INF;
+        if (jq[q] >= 0) {
+          const int32_t ej = s_beg[jq[q]] + (base + q * BLOCK + tid - s_off[jq[q]]);
+          uq[q] = (uint32_t)c.col[ej];
+          gq[q] = s_cost[jq[q]] + c.wt[ej];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < EPL; ++q)
+        if (jq[q] >= 0)
+          dq[q] = *reinterpret_cast<const int2*>(
+              &c.T[2 * (s_dst[jq[q]] + (int64_t)uq[q] * s_rows[jq[q]])]);
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        uint32_t want = 0;
+        uint64_t item = 0;
+        if (jq[q] >= 0) {
+          const int j = jq[q];
+          const uint32_t u = uq[q];
+          const int32_t gc = gq[q];
+          const int64_t dbase = s_dst[j];
+          const int64_t uoff = (int64_t)u * s_rows[j];
+          int32_t* dst = &c.T[2 * (dbase + uoff)];
+          const int2 dl = dq[q];
+          ++nRg;
+          if (gc < dl.x) {
+            const uint32_t meta = s_meta[j];
+            const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
+            int32_t best = INF;
+            bool keep = true;
+            if (!HOP || c.hop_prune) {  // completion bound before the atomic, as for merges
+              best = best_of(&c.bestw[sl]);
+              keep = dl.y < INF && (best >= INF || gc + dl.y <= (HOP ? best : best + 2));
             }
-            const int64_t erow = dbase - Sj.t_off + Sj.e_off;
-            if (atomicMax(&c.E[erow + uoff], next) < next) {
-              want = nchunks(c, u);
-              item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+            if (keep && gc < atomicMin(dst, gc)) {
+              ++nU;
+              const SlotDesc Sj = c.slots[sl];
+              if (!HOP) {
+                ++nLb;
+                const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)u * CROW],
+                                           (uint32_t)Sj.full ^ pd, c.sched[sl].R);
+                if (sumc < INF && gc + sumc < best)
+                  ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
+              }
+              const int64_t erow = dbase - Sj.t_off + Sj.e_off;
+              if (atomicMax(&c.E[erow + uoff], next) < next) {
+                want = nchunks(c, u);
+                item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+              }
             }
           }
         }
+        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
       }
-      wave_append(want, item, w.out, w.out_cnt, c.cap, c.ctr);
     }
     __syncthreads();
   }
+  stage_flush(stage, st_n, w.out, w.out_cnt, c.cap, c.ctr);
   flush(c.ctr, C_F, nF);
   flush(c.ctr, C_RG, nRg);
   flush(c.ctr, C_RM, nRm);
