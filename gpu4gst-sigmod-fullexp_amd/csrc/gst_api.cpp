@@ -110,7 +110,7 @@ struct gst_graph {
   int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
   int32_t maxw = 1;
   // engine buffers (grow-only)
-  DBuf T, Es, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
+  DBuf T, Es, GMs, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
   DBuf out_uv, out_eid, out_n, out_err;
   uint32_t* h_cnt = nullptr;  // pinned
   uint32_t wl_cap = 0;
@@ -155,7 +155,7 @@ constexpr double kWorklistFraction = 0.04;
 // Table footprint of one slot (int32 elements) and its worklist share.
 int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
   const int64_t rows = rows_of(k, D);
-  return 2 * rows * g->Vp + rows * g->Vp + (int64_t)g->Vp * CROW + RTABLE;
+  return 2 * rows * g->Vp + rows * g->Vp + g->Vp + (int64_t)g->Vp * CROW + RTABLE;
 }
 
 int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
@@ -172,6 +172,7 @@ DevCtx make_ctx(gst_graph* g) {
   c.hop_prune = g->hop_prune;
   c.T = g->T.as<int32_t>();
   c.E = g->Es.as<int32_t>();
+  c.GM = g->GMs.as<uint32_t>();
   c.C = g->C.as<int32_t>();
   c.R = g->R.as<int32_t>();
   c.vv = g->VVb.as<int32_t>();
@@ -316,6 +317,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   if (e_grew) HIP_TRY(hipMemsetAsync(g->Es.p, 0, g->Es.bytes, g->stream));
   grew |= e_grew;
   HIP_TRY(g->C.ensure((size_t)nslots * g->Vp * CROW * 4, &grew));
+  HIP_TRY(g->GMs.ensure((size_t)nslots * g->Vp * 4, &grew));
   HIP_TRY(g->R.ensure((size_t)nslots * RTABLE * 4, &grew));
   HIP_TRY(g->VVb.ensure((size_t)nslots * VV * 4, &grew));
   HIP_TRY(g->bestw.ensure((size_t)nslots * 8, &grew));
@@ -851,7 +853,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* 
 void gst_graph_destroy(gst_graph* g) {
   if (!g) return;
   g->drop_graphs();
-  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es,
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->GMs,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
                   &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();

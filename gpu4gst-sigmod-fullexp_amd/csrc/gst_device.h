@@ -54,8 +54,8 @@ __host__ __device__ inline uint32_t wit_onelabel(uint32_t v, uint32_t p) {
 struct SlotDesc {
   int64_t t_off;   // state words, vertex-major: entry v * nrows + d * nsub + p holds
                    // {cost, completion bound} as two int32, so all states of one vertex
-                   // (every merge partner) share a few cache lines; the cost word of
-                   // row 0 holds the group mask of v
+                   // (every merge partner) share a few cache lines; row 0 (p = 0) is
+                   // no state and never read
   int64_t e_off;   // enqueue stamps of DP states (iteration numbers), same indexing as T
   int64_t c_off;   // one-label distances c[v * 8 + i]
   int64_t r_off;   // route tables [p][72]
@@ -113,6 +113,7 @@ struct DevCtx {
   int32_t hop_prune;
   int32_t* T;
   int32_t* E;
+  uint32_t* GM;                  // query-group mask of each vertex [slot][Vp]
   int32_t* C;
   int32_t* R;
   int32_t* vv;

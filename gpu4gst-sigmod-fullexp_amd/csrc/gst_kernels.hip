@@ -339,13 +339,6 @@ __global__ void k_fill_u64(uint64_t* p, int64_t n, uint64_t val) {
     p[i] = val;
 }
 
-// Row 0 of every slot's table holds the query-group mask of each vertex.
-__global__ void k_zero_row0(DevCtx c) {
-  const SlotDesc S = c.slots[blockIdx.y];
-  for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.Vp; v += gridDim.x * blockDim.x)
-    c.T[2 * t_ix(S, 0, v)] = 0;
-}
-
 void launch_fill_i32(int32_t* p, int64_t n, int32_t val, hipStream_t st) {
   if (n <= 0) return;
   int64_t blocks = (n + 255) / 256;
@@ -361,7 +354,7 @@ void launch_fill_u64(uint64_t* p, int64_t n, uint64_t val, hipStream_t st) {
 }
 
 // The state words of rows p >= 1 are written whole by k_lb_table ({INF,
-// bound}) before the DP reads them, so only row 0 (group masks) is cleared here.
+// bound}) before the DP reads them, so the state table needs no clearing.
 __global__ void k_init_crows(int32_t* C, int64_t rows) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < rows;
        r += (int64_t)gridDim.x * blockDim.x) {
@@ -377,8 +370,7 @@ void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipSt
   hipLaunchKernelGGL(k_init_crows, (int)(blocks < 4096 ? blocks : 4096), 256, 0, st, c.C, rows);
   launch_fill_i32(c.vv, (int64_t)nslots * VV, INF, st);
   launch_fill_u64(c.bestw, nslots, ((uint64_t)INF << 32) | 0xFFFFFFFFull, st);
-  dim3 grid((c.Vp + 255) / 256 < 1024 ? (c.Vp + 255) / 256 : 1024, nslots);
-  hipLaunchKernelGGL(k_zero_row0, grid, 256, 0, st, c);
+  launch_fill_i32(reinterpret_cast<int32_t*>(c.GM), (int64_t)nslots * c.Vp, 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -398,7 +390,7 @@ __global__ __launch_bounds__(BLOCK) void k_groups_setup(DevCtx c, const Task* ta
       uint32_t v = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        atomicOr(&c.T[2 * t_ix(S, 0, v)], 1 << tk.bit);
+        atomicOr(&c.GM[(int64_t)tk.slot * c.Vp + v], 1u << tk.bit);
         atomicMin(&c.C[S.c_off + (int64_t)v * CROW + tk.bit], 0);
         int32_t* sw = &c.C[S.c_off + (int64_t)v * CROW + CSTAMP];
         const uint32_t old = __hip_atomic_load(reinterpret_cast<uint32_t*>(sw), __ATOMIC_RELAXED,
@@ -741,72 +733,86 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
-  // A workgroup owns 64 consecutive vertices of one slot. Each wave takes
-  // every fourth subset p, so U = full ^ p is the same in all its lanes (one
-  // lane per vertex): the loops over the groups of U are scalar branches and
-  // the route table entries scalar loads. The bounds go to an LDS tile
-  // [vertex][p], which the workgroup then writes out as whole {INF, bound}
-  // state words, contiguous in the vertex-major table.
+  // A workgroup walks tiles of 64 consecutive vertices of one slot. Each wave
+  // takes every fourth subset p, so U = full ^ p is the same in all its
+  // lanes (one lane per vertex): U's route row is read once into scalar
+  // registers, and the unrolled loops over the groups of U are scalar
+  // branches with register operands. The bounds go to an LDS tile
+  // [vertex][p], which the workgroup writes out as whole {INF, bound} state
+  // words, contiguous in the vertex-major table.
   constexpr int TV = 64;
-  __shared__ int32_t tile[TV * 128];
+  constexpr int TS = 129;  // odd tile stride: the lanes' stores hit distinct banks
+  __shared__ int32_t tile[TV * TS];
   const int s = blockIdx.y;
   const SlotDesc S = c.slots[s];
-  const int64_t v0 = (int64_t)blockIdx.x * TV;
-  if (v0 >= c.V) return;
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = threadIdx.x / WAVE;
-  const int64_t v = v0 + lane;
-  int32_t cs[8];
-  {
-    const int64_t vr = v < c.V ? v : v0;
-    const int4 a = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW]);
-    const int4 b = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW + 4]);
-    cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
-    cs[4] = b.x; cs[5] = b.y; cs[6] = b.z; cs[7] = b.w;
-  }
-  const int32_t* Rs = c.R + S.r_off;
-  for (int p = 1 + wv; p < S.nsub; p += BLOCK / WAVE) {
-    const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
-    int32_t lb = 0;
-    if (U) {
-      const int32_t* R = Rs + U * RSTRIDE;
-      int32_t one = 0, mn = INF;
-      for (int i = 0; i < 8; ++i)
-        if ((U >> i) & 1u) {
-          one = max(one, cs[i]);
-          mn = min(mn, cs[i]);
-        }
-      if (one >= INF) {
-        lb = one;
-      } else {
-        int32_t lb1 = INF, lb2 = 0;
-        for (int i = 0; i < 8; ++i) {
-          if (!((U >> i) & 1u)) continue;
-          for (int j = 0; j < 8; ++j)
-            if ((U >> j) & 1u) lb1 = min(lb1, (cs[i] + R[i * 8 + j] + cs[j]) >> 1);
-          lb2 = max(lb2, (cs[i] + R[64 + i] + mn) >> 1);
-        }
-        lb = max(one, max(lb1, lb2));
-      }
-    }
-    tile[lane * 128 + p] = lb;
-  }
-  __syncthreads();
-  const int nv = (int)min<int64_t>(TV, c.V - v0);
   const int nrows = S.nrows;
   int2* T2 = reinterpret_cast<int2*>(c.T);
-  for (int x = threadIdx.x; x < nv * nrows; x += BLOCK) {
-    const int vl = x / nrows, row = x - vl * nrows;
-    if (row == 0) continue;  // row 0 holds the group mask
-    const int p = row & (S.nsub - 1);
-    T2[t_ix(S, row, v0 + vl)] = make_int2(INF, p ? tile[vl * 128 + p] : 0);
+  for (int64_t v0 = (int64_t)blockIdx.x * TV; v0 < c.V; v0 += (int64_t)gridDim.x * TV) {
+    const int64_t v = v0 + lane;
+    int32_t cs[8];
+    {
+      const int64_t vr = v < c.V ? v : v0;
+      const int4 a = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW]);
+      const int4 b = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW + 4]);
+      cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
+      cs[4] = b.x; cs[5] = b.y; cs[6] = b.z; cs[7] = b.w;
+    }
+    for (int p = 1 + wv; p < S.nsub; p += BLOCK / WAVE) {
+      const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
+      int32_t lb = 0;
+      if (U) {
+        const int32_t* R = c.R + S.r_off + (int64_t)U * RSTRIDE;
+        int32_t r[RSTRIDE];
+#pragma unroll
+        for (int x = 0; x < RSTRIDE; ++x) r[x] = R[x];
+        int32_t one = 0, mn = INF;
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          if ((U >> i) & 1u) {
+            one = max(one, cs[i]);
+            mn = min(mn, cs[i]);
+          }
+        if (one >= INF) {
+          lb = one;
+        } else {
+          int32_t lb1 = INF, lb2 = 0;
+#pragma unroll
+          for (int i = 0; i < 8; ++i) {
+            if (!((U >> i) & 1u)) continue;
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+              if ((U >> j) & 1u) lb1 = min(lb1, (cs[i] + r[i * 8 + j] + cs[j]) >> 1);
+            lb2 = max(lb2, (cs[i] + r[64 + i] + mn) >> 1);
+          }
+          lb = max(one, max(lb1, lb2));
+        }
+      }
+      tile[lane * TS + p] = lb;
+    }
+    __syncthreads();
+    const int nv = (int)min<int64_t>(TV, c.V - v0);
+    const bool pow2 = (nrows & (nrows - 1)) == 0;  // plain slots: nrows = 2^k
+    const int sh = __builtin_ctz((unsigned)nrows);
+    for (int x = threadIdx.x; x < nv * nrows; x += BLOCK) {
+      const int vl = pow2 ? x >> sh : x / nrows, row = x - vl * nrows;
+      const int p = row & (S.nsub - 1);
+      if (p == 0) continue;  // p = 0 is no state
+      T2[t_ix(S, row, v0 + vl)] = make_int2(INF, tile[vl * TS + p]);
+    }
+    __syncthreads();
   }
 }
 
 void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) {
   if (nslots <= 0) return;  // must run for every batch: it initialises the state words
   (void)max_nsub;
-  dim3 grid((c.V + 63) / 64, nslots);
+  // enough workgroups to fill the device for one slot as for a thousand
+  const int tiles = (c.V + 63) / 64;
+  int per_slot = (8192 + nslots - 1) / nslots;
+  if (per_slot < 4) per_slot = 4;
+  dim3 grid(tiles < per_slot ? tiles : per_slot, nslots);
   hipLaunchKernelGGL(k_lb_table, grid, BLOCK, 0, st, c);
 }
 
@@ -828,7 +834,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
       uint32_t v = 0, gsv = 0;
       if (m < e) {
         v = (uint32_t)gmem[m];
-        gsv = (uint32_t)c.T[2 * t_ix(S, 0, v)];
+        gsv = c.GM[(int64_t)tk.slot * c.Vp + v];
         if ((uint32_t)__builtin_ctz(gsv | 0x100u) != (uint32_t)tk.bit) gsv = 0;
       }
       uint32_t p = gsv;  // enumerate non-empty submasks of gsv
@@ -1290,7 +1296,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     const int32_t b = st_b[top];
     __syncthreads();
     if (lane == 0) s_sp = top;
-    const uint32_t gs = (uint32_t)c.T[2 * t_ix(S, 0, v)];
+    const uint32_t gs = c.GM[(int64_t)s * c.Vp + v];
     __syncthreads();
     if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
     // merge splits: proper non-empty submasks p1 (hop: with layers d1, d2,
