@@ -254,7 +254,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--sssp-delta", type=int, default=-1)
-    ap.add_argument("--dp-div", type=int, default=-1)
+    ap.add_argument("--dp-div", type=int, default=-1,
+                    help="A* band divisor of the plain DP (0 = unordered, -1 = automatic)")
     ap.add_argument("--grid", type=int, default=0)
     ap.add_argument("--radius-pct", type=int, default=0)
     ap.add_argument("--max-batch", type=int, default=0,
@@ -351,7 +352,7 @@ def main():
     if args.single_chunk:
         G.set_option("chunk", args.chunk or 4096)
     if args.single_dp_div >= 0:
-        G.set_option("dp_delta_div", args.dp_div if args.dp_div >= 0 else 0)
+        G.set_option("dp_delta_div", args.dp_div)  # -1 = automatic
     dominant, other = _profiled_roofline(G, w, None, False)
     relax = st["grow_relax"] + st["merge_relax"]
 

@@ -127,6 +127,14 @@ struct DevCtx {
   int32_t sssp_delta;            // band width of the one-label phase
   int32_t dp_div;                // DP band width = max(1, best / dp_div); 0 = unordered
   int32_t radius_pct;            // one-label radius = best * radius_pct / 100 + 2
+  // Near/far schedule of the banded plain DP (far_mode = 1): states whose key
+  // cost + bound is at or past their slot's band go to the far list
+  // far[*far_sel] (count far_cnt[*far_sel]) and come back only when the near
+  // list has drained (k_far_*).
+  int32_t far_mode;
+  uint64_t* far[2];
+  uint32_t* far_cnt;             // [2]
+  int32_t* far_sel;
 };
 
 struct WL {
@@ -162,5 +170,7 @@ void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipSt
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
 void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st);
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
+void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
+                       uint32_t* near_cnt, hipStream_t st);
 
 }  // namespace gst

@@ -576,9 +576,11 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
   const int ns = *c.nslots;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
-    if (!x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
+    // near/far mode moves bands only when the near list has drained (k_far_*)
+    const bool fixed = dp && c.far_mode;
+    if (!fixed && !x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
     x.active = 0;
-    x.mindef = INF;
+    if (!fixed) x.mindef = INF;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -641,6 +643,110 @@ __global__ void k_dp_sched_init(DevCtx c) {
 
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st) {
   hipLaunchKernelGGL(k_dp_sched_init, 1, 256, 0, st, c);
+}
+
+// ---------------------------------------------------------------------------
+// Near/far refill of the banded plain DP, run when the near list has drained:
+// far items whose state can no longer beat the upper bound are dropped (the
+// same rules as k_dp_expand), every slot's band restarts at its smallest
+// remaining key, and the items inside the new band move to the near list.
+
+// true if the state of far item `it` is still worth expanding; key in *f
+__device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f) {
+  const SlotDesc S = c.slots[it_s(it)];
+  const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, it_p(it), it_v(it))]);
+  const int32_t t = tl.x;
+  if (t >= INF) return false;
+  const int32_t best = best_of(&c.bestw[it_s(it)]);
+  if (t + tl.y > best + 2 || !(t < best / 2 + 2)) return false;
+  *f = min(t + tl.y, INF - 1);
+  return true;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_far_min(DevCtx c) {
+  const int sel = *c.far_sel;
+  const uint32_t n = min(c.far_cnt[sel], c.cap);
+  const uint64_t* far = c.far[sel];
+  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
+    const uint64_t it = far[i];
+    int32_t f;
+    if (far_live(c, it, &f)) {
+      int32_t* m = &c.sched[it_s(it)].mindef;
+      if (f < *m) atomicMin(m, f);
+    }
+  }
+}
+
+// Band widths double at every refill: the first, narrow bands find a tight
+// upper bound early; the later, wide ones finish in few iterations.
+__global__ void k_far_thr(DevCtx c) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    Sched& x = c.sched[s];
+    if (x.mindef < INF) {
+      x.delta = min(x.delta, INF / 4) * 2;
+      x.thr = min(x.mindef + x.delta, INF);
+    }
+    x.mindef = INF;
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, uint32_t* near_cnt) {
+  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
+  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];
+  uint64_t* stage = s_stage[threadIdx.x / WAVE];
+  uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
+  uint32_t st_n = 0, fst_n = 0;
+  const int sel = *c.far_sel;
+  const uint32_t n = min(c.far_cnt[sel], c.cap);
+  const uint64_t* far = c.far[sel];
+  uint64_t* far_out = c.far[1 - sel];
+  uint32_t* far_out_cnt = c.far_cnt + (1 - sel);
+  const int32_t stamp = *c.iter;  // the iteration that will read the near list
+  for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t want = 0, fwant = 0;
+    uint64_t item = 0, fitem = 0;
+    if (i < n) {
+      const uint64_t it = far[i];
+      int32_t f;
+      if (far_live(c, it, &f)) {
+        const uint32_t v = it_v(it), s = it_s(it);
+        const SlotDesc S = c.slots[s];
+        if (f < c.sched[s].thr) {
+          // duplicates of a state in the far list collapse here
+          if (atomicMax(&c.E[e_ix(S, it_p(it), v)], stamp) < stamp) {
+            want = nchunks(c, v);
+            item = it;
+          }
+        } else {
+          fwant = 1;
+          fitem = it;
+        }
+      }
+    }
+    stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+    stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
+  }
+  stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
+  stage_flush(fstage, fst_n, far_out, far_out_cnt, c.cap, c.ctr);
+}
+
+__global__ void k_far_flip(DevCtx c) {
+  const int sel = *c.far_sel;
+  c.far_cnt[sel] = 0;
+  *c.far_sel = 1 - sel;
+}
+
+void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
+                       uint32_t* near_cnt, hipStream_t st) {
+  if (nslots <= 0 || n_far == 0) return;
+  const uint32_t blocks = (n_far + BLOCK - 1) / BLOCK;
+  const int grid = (int)(blocks < 8192 ? blocks : 8192);
+  hipLaunchKernelGGL(k_far_min, grid, BLOCK, 0, st, c);
+  hipLaunchKernelGGL(k_far_thr, 1, 256, 0, st, c);
+  hipLaunchKernelGGL(k_far_move, grid, BLOCK, 0, st, c, near, near_cnt);
+  hipLaunchKernelGGL(k_far_flip, 1, 1, 0, st, c);
 }
 
 // ---------------------------------------------------------------------------
@@ -901,8 +1007,14 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_rows[BLOCK];    // rows per vertex of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
   __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
+  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded plain DP)
   uint64_t* stage = s_stage[threadIdx.x / WAVE];
-  uint32_t st_n = 0;
+  uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
+  uint32_t st_n = 0, fst_n = 0;
+  const bool far_mode = !HOP && c.far_mode;
+  const int far_sel = far_mode ? *c.far_sel : 0;
+  uint64_t* far_out = c.far[far_sel];
+  uint32_t* far_out_cnt = c.far_cnt + far_sel;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
@@ -914,8 +1026,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
     int32_t deg = 0;
     uint32_t v = 0, p = 0, d = 0, s = 0, mask = 0;
     int32_t t = INF, Dq = 0;
-    uint32_t carry_cnt = 0;
-    uint64_t citem = 0;
+    uint32_t carry_cnt = 0, far_cnt_l = 0;
+    uint64_t citem = 0, far_item_l = 0;
     SlotDesc S;
     S.t_off = S.e_off = S.c_off = S.r_off = 0;
     S.k = S.full = S.D = S.nsub = S.vv_off = 0;
@@ -932,6 +1044,11 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       const int64_t row = HOP ? (int64_t)d * S.nsub + p : (int64_t)p;
       const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, row, v)]);
       t = tl.x;
+      // A* key of the state: its cost plus its completion bound. With bands
+      // (GST_OPT_DP_DELTA_DIV > 0) states are expanded in increasing key, so
+      // states that can lie on the optimum come first and the upper bound
+      // drops early. Capped below INF so every key can open a band.
+      const int32_t f = min(t + tl.y, INF - 1);
       Sched* sc = &c.sched[s];
       bool expand = t < INF;
       const int32_t best = best_of(&c.bestw[s]);
@@ -940,7 +1057,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       else if (HOP && c.hop_prune && best < INF && t + tl.y > best) expand = false;
       else if (!HOP) {
         const bool half = t < best / 2 + 2;  // 1/2 rule: costlier states are never expanded
-        if (ch == 0 && (!half || t < sc->thr)) {
+        if (ch == 0 && (!half || f < sc->thr)) {
           const uint32_t xs = (uint32_t)S.full ^ p;
           const int32_t t2 = c.T[2 * t_ix(S, xs, v)];
           if (t2 < INF && t < INF) {
@@ -952,12 +1069,20 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         }
         if (!half) expand = false;
       }
-      if (expand && t >= sc->thr) {  // not in the current cost band yet
+      if (expand && f >= sc->thr) {  // not in the current band yet
         expand = false;
-        if (t < sc->mindef) atomicMin(&sc->mindef, t);
-        if (atomicMax(&c.E[e_ix(S, row, v)], next) < next) {
-          carry_cnt = nchunks(c, v);
-          citem = mk_item(v, p, d, 0, s);
+        if (far_mode) {
+          // to the far list, once per state (chunk items are rebuilt on return)
+          if (ch == 0) {
+            far_cnt_l = 1;
+            far_item_l = mk_item(v, p, d, 0, s);
+          }
+        } else {
+          if (f < sc->mindef) atomicMin(&sc->mindef, f);
+          if (atomicMax(&c.E[e_ix(S, row, v)], next) < next) {
+            carry_cnt = nchunks(c, v);
+            citem = mk_item(v, p, d, 0, s);
+          }
         }
       }
       if (expand) {
@@ -984,6 +1109,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
     }
 
     stage_append(stage, st_n, carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr);
+    if (far_mode)
+      stage_append(fstage, fst_n, far_cnt_l, far_item_l, far_out, far_out_cnt, c.cap, c.ctr);
 
     // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
     {
@@ -1011,8 +1138,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       else p2 = (mask - 1u) & mask;       // proper submasks; the complement was done above
       bool live = p2 != 0;
       while (__any(live)) {
-        uint32_t want = 0;
-        uint64_t item = 0;
+        uint32_t want = 0, fwant = 0;
+        uint64_t item = 0, fitem = 0;
         if (live) {
           int32_t lbp = 0;
 #pragma unroll
@@ -1058,8 +1185,13 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                   // 2/3 rule (Conditional Tree Merging) for the queue only
                   const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
                   if (enq && atomicMax(&c.E[e_ix(S, drow, v)], next) < next) {
-                    want = nchunks(c, v);
-                    item = mk_item(v, P, nd, 0, s);
+                    if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[s].thr) {
+                      fwant = 1;  // past the band: wait in the far list
+                      fitem = mk_item(v, P, nd, 0, s);
+                    } else {
+                      want = nchunks(c, v);
+                      item = mk_item(v, P, nd, 0, s);
+                    }
                   }
                 }
               }
@@ -1076,6 +1208,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           live = p2 != 0;
         }
         stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
+        if (far_mode) stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
       }
     }
 
@@ -1110,8 +1243,8 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               &c.T[2 * (s_dst[jq[q]] + (int64_t)uq[q] * s_rows[jq[q]])]);
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
-        uint32_t want = 0;
-        uint64_t item = 0;
+        uint32_t want = 0, fwant = 0;
+        uint64_t item = 0, fitem = 0;
         if (jq[q] >= 0) {
           const int j = jq[q];
           const uint32_t u = uq[q];
@@ -1142,18 +1275,25 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               }
               const int64_t erow = dbase - Sj.t_off + Sj.e_off;
               if (atomicMax(&c.E[erow + uoff], next) < next) {
-                want = nchunks(c, u);
-                item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+                if (far_mode && min(gc + dl.y, INF - 1) >= c.sched[sl].thr) {
+                  fwant = 1;  // past the band: wait in the far list
+                  fitem = mk_item(u, pd, 0, 0, sl);
+                } else {
+                  want = nchunks(c, u);
+                  item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+                }
               }
             }
           }
         }
         stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
+        if (far_mode) stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
       }
     }
     __syncthreads();
   }
   stage_flush(stage, st_n, w.out, w.out_cnt, c.cap, c.ctr);
+  stage_flush(fstage, fst_n, far_out, far_out_cnt, c.cap, c.ctr);
   flush(c.ctr, C_F, nF);
   flush(c.ctr, C_RG, nRg);
   flush(c.ctr, C_RM, nRm);

@@ -105,6 +105,19 @@ def main(n_plain, n_hop):
             if refs[i] < O.INF() and got != refs[i]:
                 fails.append((what, i, qs[i].tolist(), got, int(refs[i])))
     G.close()
+    # a batch large enough for the automatic A* bands (near/far lists)
+    g, grp = workloads.small(80, 220, 10, seed_=23, whi=30, member_p=0.05)
+    G = GSTGraph.from_generated(g)
+    G.set_groups(grp)
+    rng = np.random.default_rng(29)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 36)]
+    br = G.solve_batch(qs)
+    for i, q in enumerate(qs):
+        cases += 1
+        ref = O.dpbf(g, grp, q)["cost"]
+        if ref < O.INF() and int(br.costs[i]) != ref:
+            fails.append(("banded-batch", i, q.tolist(), int(br.costs[i]), int(ref)))
+    G.close()
     print(json.dumps({"cases": cases, "fails": fails}))
 
 

@@ -138,6 +138,26 @@ def test_calls_of_different_shapes_on_one_graph(gpu):
                 assert r.cost == ref["cost"], ("hop", rnd, i, D)
 
 
+@pytest.mark.parametrize("div", [0, 4, 64, -1])
+def test_dp_band_schedules(gpu, div):
+    # A* bands with near/far lists (div > 0), unordered (0), automatic (-1):
+    # the schedule changes the work, never the cost
+    g, grp = workloads.small(200, 600, 10, seed_=31, whi=80, member_p=0.03)
+    G = _graph(g, grp)
+    G.set_option("dp_delta_div", div)
+    rng = np.random.default_rng(37)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 48)]
+    _say(f"band schedule div={div}: batch of 48 plain queries n={g.n}")
+    br = G.solve_batch(qs)
+    for i, q in enumerate(qs):
+        ref = O.dpbf(g, grp, q)
+        if ref["status"] == O.INFEASIBLE:
+            assert br.status[i] == STATUS_INFEASIBLE
+            continue
+        assert br.costs[i] == ref["cost"], (div, i, br.costs[i], ref["cost"])
+        _check_tree(g, grp, q, br.costs[i], br.tree(i))
+
+
 def test_infeasible_and_trivial(gpu):
     # two components: {0,1,2} and {3,4}
     g = gen.from_edges(5, [0, 1, 3], [1, 2, 4], [4, 5, 6])
