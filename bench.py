@@ -67,17 +67,21 @@ def cpu_threads() -> int:
     return max(1, min(n, 100))
 
 
-def cpu_sample(w, budget_s: float, gpu_costs, solve, name: str, cores: int):
+def cpu_sample(w, budget_s: float, gpu, solve, name: str, cores: int):
     """Times `solve` (an oracle function) over the first queries of the
-    workload until budget_s seconds have passed; costs are checked against
-    the GPU's."""
+    workload until budget_s seconds have passed; costs (or infeasibility) are
+    checked against the GPU's batch result `gpu`."""
     t0 = time.perf_counter()
     n, mismatches = 0, 0
     for i, q in enumerate(w.queries):
         r = solve(i, q)
         n += 1
-        if i < len(gpu_costs) and r["cost"] != int(gpu_costs[i]):
-            mismatches += 1
+        if i < len(gpu.costs):
+            gpu_infeasible = int(gpu.status[i]) != 0
+            cpu_infeasible = int(r["status"]) != 0
+            if gpu_infeasible != cpu_infeasible or (
+                    not gpu_infeasible and r["cost"] != int(gpu.costs[i])):
+                mismatches += 1
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
@@ -87,7 +91,7 @@ def cpu_sample(w, budget_s: float, gpu_costs, solve, name: str, cores: int):
             "cost_mismatches_vs_gpu": mismatches}
 
 
-def cpu_baselines(w, budget_s: float, gpu_costs, D=None):
+def cpu_baselines(w, budget_s: float, gpu, D=None):
     """(multi-core baseline, one-core baseline): restated TrimCDP-multi-core-CPU
     cpu_mt and PrunedDP++ for plain queries, restated D-TrimCDP-multi-core-CPU
     HOP_MT and D-PrunedDP++ HOP_cpu for hop-bounded ones."""
@@ -95,16 +99,16 @@ def cpu_baselines(w, budget_s: float, gpu_costs, D=None):
     import oracle as O
     t = cpu_threads()
     if D is None:
-        mt = cpu_sample(w, budget_s, gpu_costs,
+        mt = cpu_sample(w, budget_s, gpu,
                         lambda i, q: O.trimcdp_mt(w.graph, w.groups, q, t),
                         f"restated TrimCDP-multi-core-CPU ({t} threads)", t)
-        one = cpu_sample(w, budget_s, gpu_costs, lambda i, q: O.prunedpp(w.graph, w.groups, q),
+        one = cpu_sample(w, budget_s, gpu, lambda i, q: O.prunedpp(w.graph, w.groups, q),
                          "restated PrunedDP++ (binary heap, array tables, 1 thread)", 1)
     else:
-        mt = cpu_sample(w, budget_s, gpu_costs,
+        mt = cpu_sample(w, budget_s, gpu,
                         lambda i, q: O.hop_mt(w.graph, w.groups, q, int(D[i]), t),
                         f"restated D-TrimCDP-multi-core-CPU HOP_MT ({t} threads)", t)
-        one = cpu_sample(w, budget_s, gpu_costs,
+        one = cpu_sample(w, budget_s, gpu,
                          lambda i, q: O.hop(w.graph, w.groups, q, int(D[i])),
                          "restated D-PrunedDP++ HOP_cpu (1 thread)", 1)
     return mt, one
@@ -237,7 +241,7 @@ def other_workload(args):
     }
     if not args.no_cpu:
         out["cpu_baseline"], out["cpu_baseline_one_core"] = cpu_baselines(
-            w, args.cpu_seconds, base.costs, D)
+            w, args.cpu_seconds, base, D)
     print(json.dumps(out), flush=True)
 
 
@@ -386,7 +390,7 @@ def main():
     }
     if not args.no_cpu:
         out["cpu_baseline"], out["cpu_baseline_one_core"] = cpu_baselines(
-            w, args.cpu_seconds, base.costs)
+            w, args.cpu_seconds, base)
     print(json.dumps(out), flush=True)
 
 

@@ -839,62 +839,66 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
-  // A workgroup walks tiles of 64 consecutive vertices of one slot. Each wave
-  // takes every fourth subset p, so U = full ^ p is the same in all its
-  // lanes (one lane per vertex): U's route row is read once into scalar
-  // registers, and the unrolled loops over the groups of U are scalar
-  // branches with register operands. The bounds go to an LDS tile
-  // [vertex][p], which the workgroup writes out as whole {INF, bound} state
-  // words, contiguous in the vertex-major table.
+  // A workgroup stages its slot's route tables in LDS (entries for groups
+  // 0..6 only: 49 pair routes + 7 row minima per subset) and then walks tiles
+  // of 64 consecutive vertices. Each wave takes every fourth subset p, so
+  // U = full ^ p is the same in all its lanes (one lane per vertex) and every
+  // route read is an LDS broadcast. The loop over the first group i of a
+  // pair branches (wave-uniformly) on i in U; the second group j runs
+  // without branches: k_route leaves the routes of groups outside U (and of
+  // groups >= k) at INF, which can never be the minimum. The bounds go to an
+  // LDS tile [vertex][p] (odd stride: conflict-free), which the workgroup
+  // writes out as whole {INF, bound} state words, contiguous in the
+  // vertex-major table.
   constexpr int TV = 64;
-  constexpr int TS = 129;  // odd tile stride: the lanes' stores hit distinct banks
+  constexpr int TS = 129;
+  constexpr int RC = 57;  // compact route row: [i * 7 + j] for i, j < 7, then [49 + i]
   __shared__ int32_t tile[TV * TS];
+  __shared__ int32_t s_r[128 * RC];
   const int s = blockIdx.y;
   const SlotDesc S = c.slots[s];
+  for (int x = threadIdx.x; x < S.nsub * RC; x += BLOCK) {
+    const int X = x / RC, e = x - X * RC;
+    const int src = e < 49 ? (e / 7) * 8 + e % 7 : 64 + (e - 49);
+    s_r[x] = e < 56 ? c.R[S.r_off + X * RSTRIDE + src] : INF;
+  }
+  __syncthreads();
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = threadIdx.x / WAVE;
   const int nrows = S.nrows;
   int2* T2 = reinterpret_cast<int2*>(c.T);
   for (int64_t v0 = (int64_t)blockIdx.x * TV; v0 < c.V; v0 += (int64_t)gridDim.x * TV) {
     const int64_t v = v0 + lane;
-    int32_t cs[8];
+    int32_t cs[7];
     {
       const int64_t vr = v < c.V ? v : v0;
       const int4 a = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW]);
       const int4 b = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW + 4]);
       cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
-      cs[4] = b.x; cs[5] = b.y; cs[6] = b.z; cs[7] = b.w;
+      cs[4] = b.x; cs[5] = b.y; cs[6] = b.z;  // word 7 is the one-label stamp
     }
     for (int p = 1 + wv; p < S.nsub; p += BLOCK / WAVE) {
       const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
-      int32_t lb = 0;
-      if (U) {
-        const int32_t* R = c.R + S.r_off + (int64_t)U * RSTRIDE;
-        int32_t r[RSTRIDE];
+      const int32_t* R = s_r + U * RC;
+      int32_t one = 0, mn = INF;
 #pragma unroll
-        for (int x = 0; x < RSTRIDE; ++x) r[x] = R[x];
-        int32_t one = 0, mn = INF;
-#pragma unroll
-        for (int i = 0; i < 8; ++i)
-          if ((U >> i) & 1u) {
-            one = max(one, cs[i]);
-            mn = min(mn, cs[i]);
-          }
-        if (one >= INF) {
-          lb = one;
-        } else {
-          int32_t lb1 = INF, lb2 = 0;
-#pragma unroll
-          for (int i = 0; i < 8; ++i) {
-            if (!((U >> i) & 1u)) continue;
-#pragma unroll
-            for (int j = 0; j < 8; ++j)
-              if ((U >> j) & 1u) lb1 = min(lb1, (cs[i] + r[i * 8 + j] + cs[j]) >> 1);
-            lb2 = max(lb2, (cs[i] + r[64 + i] + mn) >> 1);
-          }
-          lb = max(one, max(lb1, lb2));
-        }
+      for (int i = 0; i < 7; ++i) {
+        const bool in = (U >> i) & 1u;
+        one = max(one, in ? cs[i] : 0);
+        mn = min(mn, in ? cs[i] : INF);
       }
+      int32_t lb1 = INF, lb2 = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) {
+        if (!((U >> i) & 1u)) continue;  // wave-uniform
+        // (c_i + R_ij + c_j) / 2 is monotone in R_ij + c_j: take the minimum first
+        int32_t m = INF;
+#pragma unroll
+        for (int j = 0; j < 7; ++j) m = min(m, R[i * 7 + j] + cs[j]);
+        lb1 = min(lb1, (cs[i] + m) >> 1);
+        lb2 = max(lb2, (cs[i] + R[49 + i] + mn) >> 1);
+      }
+      const int32_t lb = U == 0 ? 0 : (one >= INF ? one : max(one, max(lb1, lb2)));
       tile[lane * TS + p] = lb;
     }
     __syncthreads();
