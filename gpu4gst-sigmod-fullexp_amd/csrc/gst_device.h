@@ -100,7 +100,9 @@ enum Ctr {
 // (still valid lower bounds) and never enter upper bounds.
 struct Sched {
   int32_t thr, mindef, active, delta;
-  int32_t R, pad[3];
+  int32_t R;
+  int32_t prev;  // banded one-label phase: the previous band's end
+  int32_t pad[2];
 };
 
 struct DevCtx {
@@ -124,7 +126,6 @@ struct DevCtx {
   uint32_t cap;                  // worklist capacity (items)
   Sched* sched;                  // [slots]
   const int32_t* nslots;         // number of slots of the current batch (device)
-  int32_t sssp_delta;            // band width of the one-label phase
   int32_t dp_div;                // DP band width = max(1, best / dp_div); 0 = unordered
   int32_t radius_pct;            // one-label radius = best * radius_pct / 100 + 2
   // Near/far schedule of the banded plain DP (far_mode = 1): states whose key
@@ -132,6 +133,13 @@ struct DevCtx {
   // far[*far_sel] (count far_cnt[*far_sel]) and come back only when the near
   // list has drained (k_far_*).
   int32_t far_mode;
+  // Banded one-label phase (sssp_far = 1): labels at or past their slot's
+  // band end are not relaxed yet; vertices with work past the band wait in
+  // the far list (deduplicated per band with key *band_key in stamp row 0 of
+  // E, which the DP never uses) and return, flagged as continuations, when
+  // the near list has drained and the bands have grown.
+  int32_t sssp_far;
+  int32_t* band_key;
   uint64_t* far[2];
   uint32_t* far_cnt;             // [2]
   int32_t* far_sel;
@@ -172,5 +180,8 @@ void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st);
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
 void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
                        uint32_t* near_cnt, hipStream_t st);
+void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
+                            hipStream_t st);
+void launch_band_start(const DevCtx& c, hipStream_t st);
 
 }  // namespace gst

@@ -424,16 +424,24 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int64_t s_cbase[BLOCK];   // C offset of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | k << 1 | plain
   __shared__ uint32_t s_acc[2], s_abase;  // block_append
+  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded phase)
   if (threadIdx.x < 2) s_acc[threadIdx.x] = 0;
   int round = 0;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t cur = *c.iter, next = cur + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
+  const bool banded = c.sssp_far != 0;
+  const int32_t bkey = banded ? *c.band_key : 0;
+  const int far_sel = banded ? *c.far_sel : 0;
+  uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
+  uint32_t fst_n = 0;
   unsigned long long n_items = 0, n_relax = 0, n_upd = 0, n_noub = 0;
   for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
     const uint32_t idx = tile * BLOCK + tid;
     int32_t deg = 0;
+    uint32_t far_want = 0;
+    uint64_t far_item = 0;
     if (idx < n) {
       const uint64_t it = w.in[idx];
       const uint32_t v = it_v(it), ch = it_ch(it), s = it_s(it);
@@ -445,19 +453,42 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       int4 a = *reinterpret_cast<const int4*>(row);
       int4 b = *reinterpret_cast<const int4*>(row + 4);
       const uint32_t pend = pending_labels((uint32_t)b.w, (uint32_t)cur);
+      // Banded: relax only below the band end (eff); a continuation (stamp
+      // bit 7, set when the band grew) re-relaxes every label, but for a
+      // label already below the previous band end only past its old cut.
+      int32_t eff = radius, prev = 0;
+      if (banded) {
+        eff = min(radius, c.sched[s].thr);
+        prev = c.sched[s].prev;
+      }
+      const uint32_t changed = pend == 0xffu ? 0x7fu : (pend & 0x7fu);
+      const bool cont = pend != 0xffu && (pend & 0x80u);
       int32_t cv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-      int32_t lo = INF;
+      int32_t lo = INF, done_lim = INF, lo_all = INF;
+      bool beyond = false;  // a label waits for a later band
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
+        const bool live = i < S.k && cv[i] < radius;
+        if (live) lo_all = min(lo_all, cv[i]);
+        if (live && cv[i] >= eff) beyond = true;
         // unchanged labels were relaxed with these same values already
-        if (i >= S.k || cv[i] >= radius || !((pend >> i) & 1u)) cv[i] = INF;
+        const bool relax = live && cv[i] < eff && (((changed >> i) & 1u) || cont);
+        if (relax) {
+          // edges of weight <= prev - 1 - c were relaxed with this value in the
+          // previous band unless the label changed or is new to this band
+          const int32_t lim = (((changed >> i) & 1u) || cv[i] >= prev) ? -1 : prev - 1 - cv[i];
+          done_lim = min(done_lim, lim);
+        }
+        if (!relax) cv[i] = INF;
         lo = min(lo, cv[i]);
       }
+      const int32_t rb0 = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
+      const int32_t rend = min(c.rowp[v + 1], rb0 + CH);
+      int32_t re = rb0;
       if (lo < INF) {
-        // a relaxation must stay below the radius: weight <= radius - 1 - lo
-        const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
-        const int32_t re = row_cut(c.wt, rb, min(c.rowp[v + 1], rb + CH),
-                                   radius < INF ? radius - 1 - lo : INF);
+        // a relaxation must stay below the band end: weight <= eff - 1 - lo
+        const int32_t rb = done_lim < INF ? row_cut(c.wt, rb0, rend, done_lim) : rb0;
+        re = row_cut(c.wt, rb0, rend, eff < INF ? eff - 1 - lo : INF);
         deg = max(0, re - rb);
         s_beg[tid] = rb;
         s_cv[tid][0] = make_int4(cv[0], cv[1], cv[2], cv[3]);
@@ -466,7 +497,20 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         s_meta[tid] = (s << 8) | ((uint32_t)S.k << 1) | (S.D < 0 ? 1u : 0u);
         ++n_items;
       }
+      // work left for a later band: a label not yet below the band end, or
+      // edges past this band's cut that the final radius would still use
+      if (banded && eff < radius && lo_all < INF) {
+        const int32_t cut_all = lo < INF ? re : row_cut(c.wt, rb0, rend, eff - 1 - lo_all);
+        const bool more = cut_all < rend && c.wt[cut_all] <= radius - 1 - lo_all;
+        if ((beyond || more) && atomicMax(&c.E[e_ix(S, 0, v)], bkey) < bkey) {
+          far_want = 1;
+          far_item = mk_item(v, 0, 0, 0, s);
+        }
+      }
     }
+    if (banded)
+      stage_append(fstage, fst_n, far_want, far_item, c.far[far_sel], c.far_cnt + far_sel, c.cap,
+                   c.ctr);
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
     // Each lane takes EPL edges per round (strided by the block size, so
@@ -559,6 +603,8 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
     }
     __syncthreads();
   }
+  if (banded)
+    stage_flush(fstage, fst_n, c.far[far_sel], c.far_cnt + far_sel, c.cap, c.ctr);
   flush(c.ctr, C_SSSP_ITEMS, n_items);
   flush(c.ctr, C_SSSP_RELAX, n_relax);
   flush(c.ctr, C_SSSP_UPD, n_upd);
@@ -576,8 +622,8 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
   const int ns = *c.nslots;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
-    // near/far mode moves bands only when the near list has drained (k_far_*)
-    const bool fixed = dp && c.far_mode;
+    // near/far modes move bands only when the near list has drained (k_far_*)
+    const bool fixed = dp ? c.far_mode != 0 : c.sssp_far != 0;
     if (!fixed && !x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
     x.active = 0;
     if (!fixed) x.mindef = INF;
@@ -630,7 +676,7 @@ __global__ void k_dp_sched_init(DevCtx c) {
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     const SlotDesc S = c.slots[s];
     const int32_t best = best_of(&c.bestw[s]);
-    int32_t delta = c.sssp_delta;
+    int32_t delta = INF / 2;
     if (c.dp_div <= 0) delta = INF;
     else if (S.D < 0 && best < INF) delta = max(1, best / c.dp_div);
     Sched& x = c.sched[s];
@@ -746,6 +792,74 @@ void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* ne
   hipLaunchKernelGGL(k_far_min, grid, BLOCK, 0, st, c);
   hipLaunchKernelGGL(k_far_thr, 1, 256, 0, st, c);
   hipLaunchKernelGGL(k_far_move, grid, BLOCK, 0, st, c, near, near_cnt);
+  hipLaunchKernelGGL(k_far_flip, 1, 1, 0, st, c);
+}
+
+// ---------------------------------------------------------------------------
+// Banded one-label phase: when the near list has drained, every plain slot's
+// band grows (doubling, never past the radius best * pct / 100 + 2, which the
+// labels only need to reach) and the vertices waiting in the far list come
+// back as continuations (stamp bit 7); vertices of slots already at the
+// radius are dropped.
+
+__global__ void k_band_start(DevCtx c) { *c.band_key = *c.iter + 1; }
+
+void launch_band_start(const DevCtx& c, hipStream_t st) {
+  hipLaunchKernelGGL(k_band_start, 1, 1, 0, st, c);
+}
+
+__global__ void k_sssp_band_grow(DevCtx c) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    Sched& x = c.sched[s];
+    const int32_t radius = sssp_radius(c, best_of(&c.bestw[s]));
+    x.prev = x.thr;
+    if (x.thr < radius) x.thr = min(x.thr < INF / 2 ? 2 * x.thr : INF, radius);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *c.band_key = *c.iter + 1;  // the next band's first iteration + 1
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* near,
+                                                         uint32_t* near_cnt) {
+  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
+  uint64_t* stage = s_stage[threadIdx.x / WAVE];
+  uint32_t st_n = 0;
+  const int sel = *c.far_sel;
+  const uint32_t n = min(c.far_cnt[sel], c.cap);
+  const uint64_t* far = c.far[sel];
+  const uint32_t cur = (uint32_t)*c.iter;  // the iteration that will read the near list
+  for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t want = 0;
+    uint64_t item = 0;
+    if (i < n) {
+      const uint64_t it = far[i];
+      const uint32_t v = it_v(it), s = it_s(it);
+      const Sched& x = c.sched[s];
+      if (x.prev < x.thr) {  // the slot's band grew: v continues
+        const SlotDesc S = c.slots[s];
+        int32_t* sw = &c.C[S.c_off + (int64_t)v * CROW + CSTAMP];
+        const uint32_t old = __hip_atomic_load(reinterpret_cast<uint32_t*>(sw), __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+        if (mark_labels(sw, old, cur, 0x80u)) {
+          want = nchunks(c, v);
+          item = it;
+        }
+      }
+    }
+    stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+  }
+  stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
+}
+
+void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
+                            hipStream_t st) {
+  if (n_far == 0) return;
+  const uint32_t blocks = (n_far + BLOCK - 1) / BLOCK;
+  const int grid = (int)(blocks < 8192 ? blocks : 8192);
+  hipLaunchKernelGGL(k_sssp_band_grow, 1, 256, 0, st, c);
+  hipLaunchKernelGGL(k_sssp_far_move, grid, BLOCK, 0, st, c, near, near_cnt);
   hipLaunchKernelGGL(k_far_flip, 1, 1, 0, st, c);
 }
 
