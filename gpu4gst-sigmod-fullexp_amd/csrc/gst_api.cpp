@@ -155,7 +155,7 @@ int32_t rows_of(int32_t k, int32_t D) { return (D < 0 ? 1 : D + 1) << k; }
 // worklists as large as the state count, which can never overflow.
 constexpr double kWorklistFraction = 0.04;
 constexpr int kAutoBandSlots = 32;  // automatic DP bands from this many plain queries per batch
-constexpr int kAutoBandDiv = 64;    // first band width = upper bound / 64, doubling per band
+constexpr int kAutoBandDiv = 256;   // first band width = upper bound / 256 (>= 1), doubling
 constexpr int kAutoSsspBand = 16;   // first one-label band: labels below 16, doubling per band
 
 // Table footprint of one slot (int32 elements) and its worklist share.
@@ -359,8 +359,9 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     HIP_TRY(g->wl1.ensure((size_t)cap * 8, &grew));
   }
   // A* bands (near/far lists) pay off when many queries share each iteration;
-  // a lone query runs unordered (measured on the 168k-vertex workload: 1,500
-  // vs 1,360 queries/s batched, 188 vs 232 queries/s single)
+  // a lone query runs unordered (measured on the 168k-vertex workload with the
+  // unordered one-label phase: 1,500 vs 1,360 queries/s batched, 188 vs 232
+  // single; divisor 256 vs 64 with the banded one-label phase: 1,981 vs 1,915)
   g->dp_div_eff = g->dp_div >= 0 ? g->dp_div
                                  : (nslots >= kAutoBandSlots && qs[0].D < 0 ? kAutoBandDiv : 0);
   // the one-label phase of plain queries in bands of label value (delta

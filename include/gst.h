@@ -104,7 +104,7 @@ typedef struct gst_stats {
 #define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */
 #define GST_OPT_DP_DELTA_DIV 11    /* plain DP in A* bands of cost + completion bound: first band
                                       = upper bound / value, doubling; 0 = unordered; -1 (default)
-                                      = 64 for batches of 32+ plain queries, else unordered */
+                                      = 256 for batches of 32+ plain queries, else unordered */
 #define GST_OPT_SSSP_SLOTS 15      /* queries per one-label pass inside a batch (0 = all at once) */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
