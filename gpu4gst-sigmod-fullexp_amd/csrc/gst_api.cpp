@@ -110,6 +110,7 @@ struct gst_graph {
   int32_t radius_pct = 100;
   int32_t max_batch = 0;
   int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
+  int32_t list_permille = 0;  // worklist capacity override (0 = default sizing)
   int32_t maxw = 1;
   // engine buffers (grow-only)
   DBuf T, Es, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
@@ -339,6 +340,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     for (int i = 0; i < q.k; ++i) tasks.push_back(Task{s, i, q.groups[i], 0});
   }
   int64_t cap = std::max((int64_t)((double)items * wl_frac), sssp_items) + 4096;
+  if (g->list_permille > 0 && wl_frac < 1.0) cap = items * g->list_permille / 1000 + 64;
   if (cap > (int64_t)UINT32_MAX - 65536) cap = (int64_t)UINT32_MAX - 65536;
   bool grew = false;
   HIP_TRY(g->T.ensure((size_t)t_tot * 8, &grew));
@@ -556,7 +558,10 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
       ++i;
     }
     int rc = solve_lockstep(g, batch, frac, out, st);
-    if (rc == 100) rc = solve_lockstep(g, batch, 1.0, out, st);
+    if (rc == 100) {
+      if (st) ++st->reruns;
+      rc = solve_lockstep(g, batch, 1.0, out, st);
+    }
     if (rc == 100) return fail(GST_ERR_INTERNAL, "worklist overflow at full capacity");
     if (rc) return rc;
   }
@@ -782,6 +787,10 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
     case GST_OPT_SSSP_DELTA:
       if (v < -1 || v > INF / 2) return fail(GST_ERR_INVALID, "band width out of range");
       g->sssp_band = (int32_t)v;
+      break;
+    case GST_OPT_LIST_PERMILLE:
+      if (v < 0 || v > 1000) return fail(GST_ERR_INVALID, "list permille out of range");
+      g->list_permille = (int32_t)v;
       break;
     case GST_OPT_SSSP_SLOTS:
       if (v < 0 || v > (int64_t)MAX_SLOTS) return fail(GST_ERR_INVALID, "slot group out of range");

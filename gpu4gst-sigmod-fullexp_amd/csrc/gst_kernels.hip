@@ -838,14 +838,13 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* nea
       const uint32_t v = it_v(it), s = it_s(it);
       const Sched& x = c.sched[s];
       if (x.prev < x.thr) {  // the slot's band grew: v continues
+        // A vertex enters the far list at most once per band (the band key in
+        // stamp row 0) and no stamp changes while the near list is empty, so
+        // a plain store marks the continuation.
         const SlotDesc S = c.slots[s];
-        int32_t* sw = &c.C[S.c_off + (int64_t)v * CROW + CSTAMP];
-        const uint32_t old = __hip_atomic_load(reinterpret_cast<uint32_t*>(sw), __ATOMIC_RELAXED,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        if (mark_labels(sw, old, cur, 0x80u)) {
-          want = nchunks(c, v);
-          item = it;
-        }
+        c.C[S.c_off + (int64_t)v * CROW + CSTAMP] = (int32_t)((cur << 8) | 0x80u);
+        want = nchunks(c, v);
+        item = it;
       }
     }
     stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);

@@ -83,6 +83,7 @@ typedef struct gst_stats {
   double sssp_expand_ms;      /* one-label expand kernel time (profiling mode only) */
   double phase_ms[3];         /* device time: one-label phase, bounds + seeds, DP phase */
   int64_t sssp_expand_launches; /* one-label expand launches (profiling mode only) */
+  int64_t reruns;             /* batches re-run with full-size lists after a list overflowed */
 } gst_stats;
 
 /* options for gst_set_option */
@@ -106,6 +107,10 @@ typedef struct gst_stats {
                                       = upper bound / value, doubling; 0 = unordered; -1 (default)
                                       = 256 for batches of 32+ plain queries, else unordered */
 #define GST_OPT_SSSP_SLOTS 15      /* queries per one-label pass inside a batch (0 = all at once) */
+#define GST_OPT_LIST_PERMILLE 16   /* worklist capacity in permille of the batch's states, without
+                                      the usual floors (0 = default: 4%, at least one item per
+                                      vertex and query); a list that overflows makes the batch
+                                      re-run with full-size lists (gst_stats.reruns) */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
  * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge

@@ -158,6 +158,29 @@ def test_dp_band_schedules(gpu, div):
         _check_tree(g, grp, q, br.costs[i], br.tree(i))
 
 
+def test_list_overflow_reruns_exactly(gpu):
+    # lists far too small for the batch: some list (near, far, one-label or DP)
+    # overflows, the batch is re-run with full-size lists, results stay exact
+    g, grp = workloads.small(300, 900, 10, seed_=41, whi=50, member_p=0.03)
+    G = _graph(g, grp)
+    G.set_option("list_permille", 1)
+    rng = np.random.default_rng(43)
+    qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(3, 8, 40)]
+    _say(f"overflowing lists: batch of 40 plain queries n={g.n}")
+    br = G.solve_batch(qs)
+    assert br.stats["reruns"] >= 1
+    for i, q in enumerate(qs):
+        ref = O.dpbf(g, grp, q)
+        if ref["status"] == O.INFEASIBLE:
+            assert br.status[i] == STATUS_INFEASIBLE
+            continue
+        assert br.costs[i] == ref["cost"], (i, br.costs[i], ref["cost"])
+        _check_tree(g, grp, q, br.costs[i], br.tree(i))
+    _say("overflowing lists: single query")
+    r = G.solve(qs[0])
+    assert r.cost == O.dpbf(g, grp, qs[0])["cost"]
+
+
 def test_infeasible_and_trivial(gpu):
     # two components: {0,1,2} and {3,4}
     g = gen.from_edges(5, [0, 1, 3], [1, 2, 4], [4, 5, 6])
