@@ -145,6 +145,22 @@ struct gst_graph {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second.second);
     graphs.clear();
   }
+  // device bytes held by the per-batch buffers; a later batch reuses them, so
+  // they count as free when the next batch is admitted
+  size_t held_bytes() const {
+    size_t s = 0;
+    for (const DBuf* b : {&T, &Es, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
+                          &out_eid})
+      s += b->bytes;
+    return s;
+  }
+  void release_batch_buffers() {
+    drop_graphs();
+    for (DBuf* b : {&T, &Es, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv, &out_eid})
+      b->release();
+    wl_cap = 0;
+    ++gen;
+  }
 };
 
 namespace {
@@ -277,8 +293,8 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
     std::memcpy(&overflow, g->h_cnt + 2, sizeof(overflow));
     if (overflow) return 100;  // worklist too small: the batch is re-run with full capacity
     if (g->h_cnt[0] == 0) {
-      if (!((mode == 1 && c.far_mode) || (mode == 0 && c.sssp_far))) break;
-      // banded plain DP: the near list has drained; bring back the far list's next band
+      if (!((mode != 0 && c.far_mode) || (mode == 0 && c.sssp_far))) break;
+      // banded schedule: the near list has drained; bring back the far list's next band
       int32_t sel = 0;
       uint32_t fc[2] = {0, 0};
       HIP_TRY(hipMemcpyAsync(fc, cnt + 2, sizeof(fc), hipMemcpyDeviceToHost, g->stream));
@@ -371,7 +387,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   // never computed (measured on the 168k-vertex workload: 208 vs 326 ms of
   // one-label time per batch of 1,024, 3.3 G vs 7.4 G edge visits)
   g->sssp_band_eff = qs[0].D >= 0 ? 0 : (g->sssp_band >= 0 ? g->sssp_band : kAutoSsspBand);
-  const bool far_lists = qs[0].D < 0 && (g->dp_div_eff > 0 || g->sssp_band_eff > 0);
+  const bool far_lists = g->dp_div_eff > 0 || g->sssp_band_eff > 0;
   if (far_lists &&
       ((uint64_t)cap * 8 > g->far0.bytes || (uint64_t)cap * 8 > g->far1.bytes)) {
     HIP_TRY(g->far0.ensure((size_t)cap * 8, &grew));
@@ -536,7 +552,7 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
   if (budget <= 0) {
     size_t fr = 0, tot = 0;
     HIP_TRY(hipMemGetInfo(&fr, &tot));
-    budget = (int64_t)((double)fr * 0.8);
+    budget = (int64_t)((double)(fr + g->held_bytes()) * 0.8);
   }
   const double frac = kWorklistFraction;
   // stable order: plain queries first, then hop queries (one mode per batch)
@@ -559,8 +575,26 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
     }
     int rc = solve_lockstep(g, batch, frac, out, st);
     if (rc == 100) {
+      // worklists as large as the state count cannot overflow; they are
+      // (4 lists x 8 bytes per state) larger, so the batch is re-run in
+      // sub-batches that fit the budget at that size
       if (st) ++st->reruns;
-      rc = solve_lockstep(g, batch, 1.0, out, st);
+      g->release_batch_buffers();  // the held tables and the larger lists never coexist
+      size_t j = 0;
+      rc = GST_OK;
+      while (j < batch.size() && rc == GST_OK) {
+        std::vector<QueryIn> sub;
+        int64_t u = 0;
+        while (j < batch.size()) {
+          const QueryIn& q = batch[j];
+          const int64_t f = gst_query_footprint(g, q.k, q.D) + 32 * slot_items(g, q.k, q.D);
+          if (!sub.empty() && u + f > budget) break;
+          u += f;
+          sub.push_back(q);
+          ++j;
+        }
+        rc = solve_lockstep(g, sub, 1.0, out, st);
+      }
     }
     if (rc == 100) return fail(GST_ERR_INTERNAL, "worklist overflow at full capacity");
     if (rc) return rc;
@@ -820,7 +854,8 @@ int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
   if (!g || k < 1 || k > GST_MAX_K) return -1;
   const double wl_items =
       std::max(kWorklistFraction * (double)slot_items(g, k, D), (double)g->nch());
-  return slot_ints(g, k, D) * 4 + (int64_t)(2 * 8 * wl_items) +
+  // two worklists and the two far lists of the banded schedules
+  return slot_ints(g, k, D) * 4 + (int64_t)(4 * 8 * wl_items) +
          (int64_t)g->max_tree_edges * 12 + 1024;
 }
 

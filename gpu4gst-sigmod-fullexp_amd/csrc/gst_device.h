@@ -126,9 +126,11 @@ struct DevCtx {
   uint32_t cap;                  // worklist capacity (items)
   Sched* sched;                  // [slots]
   const int32_t* nslots;         // number of slots of the current batch (device)
-  int32_t dp_div;                // DP band width = max(1, best / dp_div); 0 = unordered
+  // first DP band width = max(1, best / dp_div), for a hop-bounded slot
+  // max(1, smallest key / dp_div); doubling per band; 0 = unordered
+  int32_t dp_div;
   int32_t radius_pct;            // one-label radius = best * radius_pct / 100 + 2
-  // Near/far schedule of the banded plain DP (far_mode = 1): states whose key
+  // Near/far schedule of the banded DP (far_mode = 1): states whose key
   // cost + bound is at or past their slot's band go to the far list
   // far[*far_sel] (count far_cnt[*far_sel]) and come back only when the near
   // list has drained (k_far_*).

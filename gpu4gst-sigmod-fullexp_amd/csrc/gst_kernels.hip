@@ -36,6 +36,10 @@ __device__ inline int lane_id() { return __lane_id(); }
 #define GST_EPL 4
 #endif
 constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
+#ifndef GST_MPL
+#define GST_MPL 2
+#endif
+constexpr int MPL = GST_MPL;  // merge pairs per lane per round in the DP kernel
 
 // Appends `cnt` chunk items (item | chunk << 36) per lane; every lane of the
 // wavefront must call this together.
@@ -679,6 +683,7 @@ __global__ void k_dp_sched_init(DevCtx c) {
     int32_t delta = INF / 2;
     if (c.dp_div <= 0) delta = INF;
     else if (S.D < 0 && best < INF) delta = max(1, best / c.dp_div);
+    else if (S.D >= 0) delta = 0;  // everything to the far list; the first refill sizes the band
     Sched& x = c.sched[s];
     x.delta = delta;
     x.thr = delta;
@@ -692,19 +697,28 @@ void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st) {
 }
 
 // ---------------------------------------------------------------------------
-// Near/far refill of the banded plain DP, run when the near list has drained:
+// Near/far refill of the banded DP, run when the near list has drained:
 // far items whose state can no longer beat the upper bound are dropped (the
 // same rules as k_dp_expand), every slot's band restarts at its smallest
 // remaining key, and the items inside the new band move to the near list.
 
 // true if the state of far item `it` is still worth expanding; key in *f
+// state row of a DP item: p, or d * 2^k + p for a hop-bounded slot
+__device__ inline int64_t item_row(const SlotDesc& S, uint64_t it) {
+  return S.D >= 0 ? (int64_t)it_d(it) * S.nsub + it_p(it) : (int64_t)it_p(it);
+}
+
 __device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f) {
   const SlotDesc S = c.slots[it_s(it)];
-  const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, it_p(it), it_v(it))]);
+  const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
   const int32_t t = tl.x;
   if (t >= INF) return false;
   const int32_t best = best_of(&c.bestw[it_s(it)]);
-  if (t + tl.y > best + 2 || !(t < best / 2 + 2)) return false;
+  if (S.D >= 0) {
+    if (c.hop_prune && best < INF && t + tl.y > best) return false;
+  } else if (t + tl.y > best + 2 || !(t < best / 2 + 2)) {
+    return false;
+  }
   *f = min(t + tl.y, INF - 1);
   return true;
 }
@@ -730,7 +744,9 @@ __global__ void k_far_thr(DevCtx c) {
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
     if (x.mindef < INF) {
-      x.delta = min(x.delta, INF / 4) * 2;
+      // a hop-bounded slot starts with no upper bound: its first band is
+      // sized from the smallest key (a lower bound of the optimum) instead
+      x.delta = x.delta > 0 ? min(x.delta, INF / 4) * 2 : max(1, x.mindef / c.dp_div);
       x.thr = min(x.mindef + x.delta, INF);
     }
     x.mindef = INF;
@@ -761,7 +777,7 @@ __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, ui
         const SlotDesc S = c.slots[s];
         if (f < c.sched[s].thr) {
           // duplicates of a state in the far list collapse here
-          if (atomicMax(&c.E[e_ix(S, it_p(it), v)], stamp) < stamp) {
+          if (atomicMax(&c.E[e_ix(S, item_row(S, it), v)], stamp) < stamp) {
             want = nchunks(c, v);
             item = it;
           }
@@ -1123,12 +1139,17 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row of vertex 0
   __shared__ int32_t s_rows[BLOCK];    // rows per vertex of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
+  __shared__ uint32_t s_v[BLOCK];      // merge side: the item's vertex
+  __shared__ int32_t s_mlim[BLOCK];    //   largest useful partner cost
+  __shared__ uint32_t s_mm[BLOCK];     //   mask | partner depths << 8
+  __shared__ int64_t s_tb[BLOCK];      //   T offset of the vertex's row 0
+  __shared__ uint4 s_cv[BLOCK];        //   one-label distances, 16 bits each
   __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
-  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded plain DP)
+  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded DP)
   uint64_t* stage = s_stage[threadIdx.x / WAVE];
   uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
   uint32_t st_n = 0, fst_n = 0;
-  const bool far_mode = !HOP && c.far_mode;
+  const bool far_mode = c.far_mode != 0;
   const int far_sel = far_mode ? *c.far_sel : 0;
   uint64_t* far_out = c.far[far_sel];
   uint32_t* far_out_cnt = c.far_cnt + far_sel;
@@ -1229,105 +1250,149 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
     if (far_mode)
       stage_append(fstage, fst_n, far_cnt_l, far_item_l, far_out, far_out_cnt, c.cap, c.ctr);
 
-    // ---- merges (one lane per item; wave-uniform loop so appends stay collective)
+    // ---- merges, balanced over the workgroup like the grow edges: item j
+    // owns pairs s_off[j] .. s_off[j + 1] - 1; its pair r merges with subset
+    // number r / nd2 + 1 of its mask (the bits of that number deposited into
+    // the mask's set bits) at partner depth r % nd2 (hop: nd2 = D - d + 1;
+    // plain: nd2 = 1 and the mask itself is left out, the complement having
+    // been merged above).
     {
-      // A merged tree costlier than best + 2 (hop: best) is useless, and
-      // T(v, p2) >= max_{i in p2} c_i(v), so partners whose one-label bound
-      // already exceeds the margin are skipped without reading T.
-      int32_t mlim = INF;
-      int32_t cvm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      int32_t mcnt = 0;
       if (mask) {
+        // A merged tree costlier than best + 2 (hop: best) is useless, and
+        // T(v, p2) >= max_{i in p2} c_i(v), so partners whose one-label bound
+        // already exceeds the margin are skipped without reading T. The
+        // labels are kept in LDS capped at 65535 (still lower bounds).
+        int32_t mlim = INF;
         const int32_t bnow = best_of(&c.bestw[s]);
         if (bnow < INF) {
           if (!HOP) mlim = bnow + 2 - t;
           else if (c.hop_prune) mlim = bnow - t;
         }
+        uint4 cv = make_uint4(0u, 0u, 0u, 0u);
         if (mlim < INF) {
           const int32_t* crow = &c.C[S.c_off + (int64_t)v * CROW];
           const int4 a = *reinterpret_cast<const int4*>(crow);
           const int4 b = *reinterpret_cast<const int4*>(crow + 4);
-          cvm[0] = a.x; cvm[1] = a.y; cvm[2] = a.z; cvm[3] = a.w;
-          cvm[4] = b.x; cvm[5] = b.y; cvm[6] = b.z; cvm[7] = b.w;
+          auto h = [](int32_t x) { return (uint32_t)min(x, 65535); };
+          cv = make_uint4(h(a.x) | h(a.y) << 16, h(a.z) | h(a.w) << 16, h(b.x) | h(b.y) << 16,
+                          h(b.z));
         }
+        const int32_t nd2 = HOP ? Dq - (int32_t)d + 1 : 1;
+        mcnt = ((1 << __popc(mask)) - (HOP ? 1 : 2)) * nd2;
+        s_v[tid] = v;
+        s_mlim[tid] = mlim;
+        s_mm[tid] = mask | ((uint32_t)nd2 << 8);
+        s_tb[tid] = S.t_off + (int64_t)v * S.nrows;
+        s_cv[tid] = cv;
       }
-      uint32_t p2, d2 = 0;
-      if (HOP) p2 = mask;                 // every non-empty submask, complement included
-      else p2 = (mask - 1u) & mask;       // proper submasks; the complement was done above
-      bool live = p2 != 0;
-      while (__any(live)) {
-        uint32_t want = 0, fwant = 0;
-        uint64_t item = 0, fitem = 0;
-        if (live) {
+      block_scan(mcnt, s_off, s_wsum);
+    }
+    const int32_t mtotal = s_off[BLOCK];
+    for (int32_t base = 0; base < mtotal; base += BLOCK * MPL) {
+      int jq[MPL];
+      uint32_t p2q[MPL], d2q[MPL];
+      int32_t t2q[MPL];
+#pragma unroll
+      for (int q = 0; q < MPL; ++q) {
+        const int32_t e = base + q * BLOCK + tid;
+        jq[q] = e < mtotal ? find_item(s_off, e) : -1;
+      }
+#pragma unroll
+      for (int q = 0; q < MPL; ++q) {
+        p2q[q] = 0;
+        d2q[q] = 0;
+        t2q[q] = INF;
+        if (jq[q] >= 0) {
+          const int j = jq[q];
+          const uint32_t r = (uint32_t)(base + q * BLOCK + tid - s_off[j]);
+          const uint32_t mm = s_mm[j], mk = mm & 0xFFu;
+          uint32_t idx = r + 1;
+          if (HOP) {
+            const uint32_t nd2 = mm >> 8;
+            idx = r / nd2 + 1;
+            d2q[q] = r - (idx - 1) * nd2;
+          }
+          uint32_t p2 = 0;
+          for (uint32_t rest = mk; rest; rest &= rest - 1u, idx >>= 1)
+            if (idx & 1u) p2 |= rest & (0u - rest);
+          p2q[q] = p2;
+          const uint4 cv = s_cv[j];
+          const uint32_t w4[4] = {cv.x, cv.y, cv.z, cv.w};
           int32_t lbp = 0;
 #pragma unroll
-          for (int i = 0; i < 8; ++i)
-            if ((p2 >> i) & 1u) lbp = max(lbp, cvm[i]);
-          const int64_t src = HOP ? ((int64_t)d2 * S.nsub + p2) : (int64_t)p2;
-          const int32_t t2 = lbp > mlim ? INF : c.T[2 * t_ix(S, src, v)];
-          nRm += lbp <= mlim;
-          if (t2 < INF && t2 <= mlim) {
-            const int32_t mc = t + t2;
-            const uint32_t P = p | p2;
-            const uint32_t nd = HOP ? max(d, d2) : 0u;
-            const int64_t drow = HOP ? ((int64_t)nd * S.nsub + P) : (int64_t)P;
-            int32_t* dst = &c.T[2 * t_ix(S, drow, v)];
-            const int2 dl = *reinterpret_cast<const int2*>(dst);
-            if (mc < dl.x) {
-              if ((int32_t)P == S.full) {
-                if (mc < atomicMin(dst, mc)) {
-                  ++nU;
-                  ub_update(&c.bestw[s], mc, wit_full(v, nd));
-                }
-              } else {
-                // Completion bound first: a state that cannot beat best + 2
-                // (hop: best) lies on no better solution, so its entry is
-                // left alone and no atomic is issued.
-                int32_t best = INF;
-                bool keep = true;
-                if (!HOP || c.hop_prune) {
-                  best = best_of(&c.bestw[s]);
-                  keep = dl.y < INF && (best >= INF || mc + dl.y <= (HOP ? best : best + 2));
-                }
-                if (keep && mc < atomicMin(dst, mc)) {
-                  ++nU;
-                  if (!HOP) {
-                    ++nLb;
-                    const int32_t sumc = sum_c(&c.C[S.c_off + (int64_t)v * CROW],
-                                               (uint32_t)S.full ^ P, c.sched[s].R);
-                    if (sumc < INF && mc + sumc < best) {
-                      ub_update(&c.bestw[s], mc + sumc, wit_onelabel(v, P));
-                      best = mc + sumc;
-                    }
+          for (int i = 0; i < 7; ++i)
+            if ((p2 >> i) & 1u) lbp = max(lbp, (int32_t)((w4[i >> 1] >> (16 * (i & 1))) & 0xFFFFu));
+          if (lbp <= s_mlim[j]) {
+            ++nRm;
+            const int32_t nsub = (int32_t)(mk | (s_meta[j] & 0xFFu)) + 1;
+            t2q[q] = c.T[2 * (s_tb[j] + (HOP ? (int64_t)d2q[q] * nsub : 0) + p2)];
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < MPL; ++q) {
+        uint32_t want = 0, fwant = 0;
+        uint64_t item = 0, fitem = 0;
+        const int j = jq[q];
+        if (j >= 0 && t2q[q] < INF && t2q[q] <= s_mlim[j]) {
+          const uint32_t meta = s_meta[j];
+          const uint32_t sl = meta >> 12, dj = (meta >> 8) & 0xFu, pj = meta & 0xFFu;
+          const uint32_t mk = s_mm[j] & 0xFFu, p2 = p2q[q], P = pj | p2;
+          const uint32_t vj = s_v[j];
+          const int32_t mc = s_cost[j] + t2q[q];
+          const uint32_t nd = HOP ? max(dj, d2q[q]) : 0u;
+          const int64_t drow = HOP ? (int64_t)nd * (int64_t)((mk | pj) + 1) + P : (int64_t)P;
+          int32_t* dst = &c.T[2 * (s_tb[j] + drow)];
+          const int2 dl = *reinterpret_cast<const int2*>(dst);
+          if (mc < dl.x) {
+            if (HOP && p2 == mk) {  // a full tree
+              if (mc < atomicMin(dst, mc)) {
+                ++nU;
+                ub_update(&c.bestw[sl], mc, wit_full(vj, nd));
+              }
+            } else {
+              // Completion bound first: a state that cannot beat best + 2
+              // (hop: best) lies on no better solution, so its entry is left
+              // alone and no atomic is issued.
+              int32_t best = INF;
+              bool keep = true;
+              if (!HOP || c.hop_prune) {
+                best = best_of(&c.bestw[sl]);
+                keep = dl.y < INF && (best >= INF || mc + dl.y <= (HOP ? best : best + 2));
+              }
+              if (keep && mc < atomicMin(dst, mc)) {
+                ++nU;
+                const SlotDesc Sj = c.slots[sl];
+                if (!HOP) {
+                  ++nLb;
+                  const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)vj * CROW], mk ^ p2,
+                                             c.sched[sl].R);
+                  if (sumc < INF && mc + sumc < best) {
+                    ub_update(&c.bestw[sl], mc + sumc, wit_onelabel(vj, P));
+                    best = mc + sumc;
                   }
-                  // 2/3 rule (Conditional Tree Merging) for the queue only
-                  const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
-                  if (enq && atomicMax(&c.E[e_ix(S, drow, v)], next) < next) {
-                    if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[s].thr) {
-                      fwant = 1;  // past the band: wait in the far list
-                      fitem = mk_item(v, P, nd, 0, s);
-                    } else {
-                      want = nchunks(c, v);
-                      item = mk_item(v, P, nd, 0, s);
-                    }
+                }
+                // 2/3 rule (Conditional Tree Merging) for the queue only
+                const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
+                if (enq && atomicMax(&c.E[e_ix(Sj, drow, vj)], next) < next) {
+                  if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[sl].thr) {
+                    fwant = 1;  // past the band: wait in the far list
+                    fitem = mk_item(vj, P, nd, 0, sl);
+                  } else {
+                    want = nchunks(c, vj);
+                    item = mk_item(vj, P, nd, 0, sl);
                   }
                 }
               }
             }
           }
-          if (HOP) {
-            if ((int32_t)++d2 > Dq - (int32_t)d) {
-              d2 = 0;
-              p2 = (p2 - 1u) & mask;
-            }
-          } else {
-            p2 = (p2 - 1u) & mask;
-          }
-          live = p2 != 0;
         }
         stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
         if (far_mode) stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
       }
     }
+    __syncthreads();  // s_off is reused by the grow scan
 
     // ---- grow: the workgroup's edges, balanced by an LDS prefix scan
     block_scan(deg, s_off, s_wsum);
@@ -1394,7 +1459,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               if (atomicMax(&c.E[erow + uoff], next) < next) {
                 if (far_mode && min(gc + dl.y, INF - 1) >= c.sched[sl].thr) {
                   fwant = 1;  // past the band: wait in the far list
-                  fitem = mk_item(u, pd, 0, 0, sl);
+                  fitem = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
                 } else {
                   want = nchunks(c, u);
                   item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);

@@ -140,8 +140,11 @@ int gst_solve(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D, i
 
 /* nq queries in one call: query q is q_groups[q_offsets[q] .. q_offsets[q+1]),
  * with hop bound q_D[q] (q_D may be NULL = all plain). Queries are admitted in
- * batches whose device footprint fits hbm_budget_bytes (0 = 80% of free HBM)
- * and solved concurrently. Trees go to out_edge_uv (pairs) / out_edge_ids
+ * batches whose device footprint fits hbm_budget_bytes (0 = 80% of the free
+ * HBM plus what the graph's per-batch buffers already hold) and solved
+ * concurrently. A batch whose worklists overflow is solved again in
+ * sub-batches with worklists as large as the state count. Trees go to
+ * out_edge_uv (pairs) / out_edge_ids
  * (undirected ids, may be NULL), query q at [out_edge_offsets[q],
  * out_edge_offsets[q+1]). If edge_cap is too small GST_ERR_CAPACITY is
  * returned after costs and offsets are written. */

@@ -158,17 +158,42 @@ def test_dp_band_schedules(gpu, div):
         _check_tree(g, grp, q, br.costs[i], br.tree(i))
 
 
+@pytest.mark.parametrize("div", [0, 4, 64])
+def test_hop_band_schedules(gpu, div):
+    # hop-bounded queries under A* bands (the first band sized from the
+    # smallest key, no upper bound being known) and unordered
+    g, grp = workloads.small(150, 420, 8, seed_=47, whi=60, member_p=0.04)
+    G = _graph(g, grp)
+    G.set_option("dp_delta_div", div)
+    rng = np.random.default_rng(53)
+    qs = [rng.choice(8, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 7, 40)]
+    D = rng.integers(1, 5, 40).astype(np.int32)
+    _say(f"hop band schedule div={div}: batch of 40 hop queries n={g.n}")
+    br = G.solve_batch(qs, D=D)
+    for i, q in enumerate(qs):
+        ref = O.hop(g, grp, q, int(D[i]))
+        if ref["status"] == O.INFEASIBLE:
+            assert br.status[i] == STATUS_INFEASIBLE
+            continue
+        assert br.costs[i] == ref["cost"], (div, i, br.costs[i], ref["cost"])
+        _check_tree(g, grp, q, br.costs[i], br.tree(i), int(D[i]))
+
+
 def test_list_overflow_reruns_exactly(gpu):
     # lists far too small for the batch: some list (near, far, one-label or DP)
-    # overflows, the batch is re-run with full-size lists, results stay exact
+    # overflows, the batch is re-run with full-size lists, results stay exact;
+    # the budget admits the batch at once but not its full-size re-run, which
+    # goes in sub-batches
     g, grp = workloads.small(300, 900, 10, seed_=41, whi=50, member_p=0.03)
     G = _graph(g, grp)
     G.set_option("list_permille", 1)
     rng = np.random.default_rng(43)
     qs = [rng.choice(10, int(k), replace=False).astype(np.int32) for k in rng.integers(3, 8, 40)]
-    _say(f"overflowing lists: batch of 40 plain queries n={g.n}")
-    br = G.solve_batch(qs)
+    budget = sum(G.footprint(len(q)) for q in qs)
+    _say(f"overflowing lists: batch of 40 plain queries n={g.n} budget={budget}")
+    br = G.solve_batch(qs, hbm_budget=budget)
     assert br.stats["reruns"] >= 1
+    assert br.stats["batches"] >= 3
     for i, q in enumerate(qs):
         ref = O.dpbf(g, grp, q)
         if ref["status"] == O.INFEASIBLE:
