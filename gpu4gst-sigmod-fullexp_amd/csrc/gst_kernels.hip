@@ -36,6 +36,13 @@ __device__ inline int lane_id() { return __lane_id(); }
 #define GST_EPL 4
 #endif
 constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
+// occupancy hints of the two expand kernels (empty by default)
+#ifndef GST_SSSP_ATTR
+#define GST_SSSP_ATTR
+#endif
+#ifndef GST_DP_ATTR
+#define GST_DP_ATTR
+#endif
 #ifndef GST_MPL
 #define GST_MPL 2
 #endif
@@ -77,7 +84,10 @@ __device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, u
 // wave's buffer and go to the list with one global atomicAdd per WSTAGE
 // items (the list counter is a single word every wave of the grid would
 // otherwise hit once per append round). `n` is the wave-uniform fill level.
-constexpr int WSTAGE = 256;
+#ifndef GST_WSTAGE
+#define GST_WSTAGE 256
+#endif
+constexpr int WSTAGE = GST_WSTAGE;
 
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -420,7 +430,7 @@ void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const i
 // single row access at u. Plain queries drop labels at or beyond the running
 // upper bound + 2 (see k_clamp).
 
-__global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
+__global__ __launch_bounds__(BLOCK) GST_SSSP_ATTR void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
@@ -967,25 +977,30 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // groups outside p (the reference's lb0 table, DPQ.cu:193-243, evaluated here
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
+#ifndef GST_LB_TP
+#define GST_LB_TP 32
+#endif
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   // A workgroup stages its slot's route tables in LDS (entries for groups
   // 0..6 only: 49 pair routes + 7 row minima per subset) and then walks tiles
-  // of 64 consecutive vertices. Each wave takes every fourth subset p, so
-  // U = full ^ p is the same in all its lanes (one lane per vertex) and every
-  // route read is an LDS broadcast. The loop over the first group i of a
-  // pair branches (wave-uniformly) on i in U; the second group j runs
-  // without branches: k_route leaves the routes of groups outside U (and of
-  // groups >= k) at INF, which can never be the minimum. The bounds go to an
-  // LDS tile [vertex][p] (odd stride: conflict-free), which the workgroup
-  // writes out as whole {INF, bound} state words, contiguous in the
-  // vertex-major table.
+  // of 64 consecutive vertices, subsets in passes of TP (a smaller tile than
+  // all 128 subsets at once lets four workgroups share a CU). Each wave
+  // takes every fourth subset p, so U = full ^ p is the same in all its lanes
+  // (one lane per vertex) and every route read is an LDS broadcast. The loop
+  // over the first group i of a pair branches (wave-uniformly) on i in U; the
+  // second group j runs without branches: k_route leaves the routes of groups
+  // outside U (and of groups >= k) at INF, which can never be the minimum.
+  // The bounds go to an LDS tile [vertex][p] (odd stride: conflict-free),
+  // which the workgroup writes out as whole {INF, bound} state words,
+  // contiguous in the vertex-major table.
   constexpr int TV = 64;
-  constexpr int TS = 129;
+  constexpr int TP = GST_LB_TP;
+  constexpr int TS = TP + 1;
   constexpr int RC = 57;  // compact route row: [i * 7 + j] for i, j < 7, then [49 + i]
   __shared__ int32_t tile[TV * TS];
-  __shared__ int32_t s_r[128 * RC];
   const int s = blockIdx.y;
   const SlotDesc S = c.slots[s];
+  __shared__ int32_t s_r[128 * RC];
   for (int x = threadIdx.x; x < S.nsub * RC; x += BLOCK) {
     const int X = x / RC, e = x - X * RC;
     const int src = e < 49 ? (e / 7) * 8 + e % 7 : 64 + (e - 49);
@@ -995,6 +1010,7 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   const int lane = threadIdx.x & (WAVE - 1);
   const int wv = threadIdx.x / WAVE;
   const int nrows = S.nrows;
+  const int nd = nrows / S.nsub;  // hop: D + 1 copies of the subsets
   int2* T2 = reinterpret_cast<int2*>(c.T);
   for (int64_t v0 = (int64_t)blockIdx.x * TV; v0 < c.V; v0 += (int64_t)gridDim.x * TV) {
     const int64_t v = v0 + lane;
@@ -1006,41 +1022,47 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
       cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
       cs[4] = b.x; cs[5] = b.y; cs[6] = b.z;  // word 7 is the one-label stamp
     }
-    for (int p = 1 + wv; p < S.nsub; p += BLOCK / WAVE) {
-      const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
-      const int32_t* R = s_r + U * RC;
-      int32_t one = 0, mn = INF;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        const bool in = (U >> i) & 1u;
-        one = max(one, in ? cs[i] : 0);
-        mn = min(mn, in ? cs[i] : INF);
-      }
-      int32_t lb1 = INF, lb2 = 0;
-#pragma unroll
-      for (int i = 0; i < 7; ++i) {
-        if (!((U >> i) & 1u)) continue;  // wave-uniform
-        // (c_i + R_ij + c_j) / 2 is monotone in R_ij + c_j: take the minimum first
-        int32_t m = INF;
-#pragma unroll
-        for (int j = 0; j < 7; ++j) m = min(m, R[i * 7 + j] + cs[j]);
-        lb1 = min(lb1, (cs[i] + m) >> 1);
-        lb2 = max(lb2, (cs[i] + R[49 + i] + mn) >> 1);
-      }
-      const int32_t lb = U == 0 ? 0 : (one >= INF ? one : max(one, max(lb1, lb2)));
-      tile[lane * TS + p] = lb;
-    }
-    __syncthreads();
     const int nv = (int)min<int64_t>(TV, c.V - v0);
-    const bool pow2 = (nrows & (nrows - 1)) == 0;  // plain slots: nrows = 2^k
-    const int sh = __builtin_ctz((unsigned)nrows);
-    for (int x = threadIdx.x; x < nv * nrows; x += BLOCK) {
-      const int vl = pow2 ? x >> sh : x / nrows, row = x - vl * nrows;
-      const int p = row & (S.nsub - 1);
-      if (p == 0) continue;  // p = 0 is no state
-      T2[t_ix(S, row, v0 + vl)] = make_int2(INF, tile[vl * TS + p]);
+    for (int h = 0; h < S.nsub; h += TP) {
+      const int np = min(TP, S.nsub - h);  // a power of two
+      for (int p = h + wv; p < h + np; p += BLOCK / WAVE) {
+        if (p == 0) continue;  // p = 0 is no state
+        const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
+        const int32_t* R = s_r + U * RC;
+        int32_t one = 0, mn = INF;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          const bool in = (U >> i) & 1u;
+          one = max(one, in ? cs[i] : 0);
+          mn = min(mn, in ? cs[i] : INF);
+        }
+        int32_t lb1 = INF, lb2 = 0;
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+          if (!((U >> i) & 1u)) continue;  // wave-uniform
+          // (c_i + R_ij + c_j) / 2 is monotone in R_ij + c_j: take the minimum first
+          int32_t m = INF;
+#pragma unroll
+          for (int j = 0; j < 7; ++j) m = min(m, R[i * 7 + j] + cs[j]);
+          lb1 = min(lb1, (cs[i] + m) >> 1);
+          lb2 = max(lb2, (cs[i] + R[49 + i] + mn) >> 1);
+        }
+        const int32_t lb = U == 0 ? 0 : (one >= INF ? one : max(one, max(lb1, lb2)));
+        tile[lane * TS + (p - h)] = lb;
+      }
+      __syncthreads();
+      const int sh = __builtin_ctz((unsigned)np);
+      const int per_v = nd << sh;
+      for (int x = threadIdx.x; x < nv * per_v; x += BLOCK) {
+        const int vl = nd == 1 ? x >> sh : x / per_v;
+        const int rem = x - vl * per_v;
+        const int pp = rem & (np - 1);
+        if (h + pp == 0) continue;  // p = 0 is no state
+        const int row = (rem >> sh) * S.nsub + h + pp;
+        T2[t_ix(S, row, v0 + vl)] = make_int2(INF, tile[vl * TS + pp]);
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -1131,7 +1153,7 @@ void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_
 //     (lb from hop-free distances is a valid bound for hop-bounded trees)
 
 template <bool HOP>
-__global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
+__global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
