@@ -382,11 +382,13 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   // single; divisor 256 vs 64 with the banded one-label phase: 1,981 vs 1,915)
   g->dp_div_eff = g->dp_div >= 0 ? g->dp_div
                                  : (nslots >= kAutoBandSlots && qs[0].D < 0 ? kAutoBandDiv : 0);
-  // the one-label phase of plain queries in bands of label value (delta
-  // stepping with near/far lists), so that labels past the final radius are
-  // never computed (measured on the 168k-vertex workload: 208 vs 326 ms of
-  // one-label time per batch of 1,024, 3.3 G vs 7.4 G edge visits)
-  g->sssp_band_eff = qs[0].D >= 0 ? 0 : (g->sssp_band >= 0 ? g->sssp_band : kAutoSsspBand);
+  // the one-label phase in bands of label value (delta stepping with near/far
+  // lists), so that labels past the final radius are never computed and
+  // label-correcting churn collapses (measured on the 168k-vertex workload:
+  // 208 vs 326 ms of one-label time per batch of 1,024, 3.3 G vs 7.4 G edge
+  // visits; hop-bounded queries, which have no radius, on the 978k-vertex
+  // workload: 619 vs 720 ms, 8.5 G vs 15.2 G edge visits)
+  g->sssp_band_eff = g->sssp_band >= 0 ? g->sssp_band : kAutoSsspBand;
   const bool far_lists = g->dp_div_eff > 0 || g->sssp_band_eff > 0;
   if (far_lists &&
       ((uint64_t)cap * 8 > g->far0.bytes || (uint64_t)cap * 8 > g->far1.bytes)) {

@@ -822,11 +822,12 @@ void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* ne
 }
 
 // ---------------------------------------------------------------------------
-// Banded one-label phase: when the near list has drained, every plain slot's
-// band grows (doubling, never past the radius best * pct / 100 + 2, which the
-// labels only need to reach) and the vertices waiting in the far list come
-// back as continuations (stamp bit 7); vertices of slots already at the
-// radius are dropped.
+// Banded one-label phase: when the near list has drained, every slot's band
+// grows (doubling, never past the radius best * pct / 100 + 2, which the
+// labels only need to reach; a hop-bounded slot has no radius and grows until
+// no vertex waits) and the vertices waiting in the far list come back as
+// continuations (stamp bit 7); vertices of slots already at the radius are
+// dropped.
 
 __global__ void k_band_start(DevCtx c) { *c.band_key = *c.iter + 1; }
 

@@ -97,9 +97,9 @@ typedef struct gst_stats {
                                       100000 reproduces the reference's inf) */
 #define GST_OPT_MAX_TREE_EDGES 7   /* per-query edge capacity of the backtracking kernel */
 #define GST_OPT_USE_GRAPHS 8       /* 1 = capture iteration steps in HIP graphs (default 1) */
-#define GST_OPT_SSSP_DELTA 10      /* one-label phase of plain batches in bands of label value:
-                                      first band = value, doubling up to the radius; 0 = unordered;
-                                      -1 (default) = 16 */
+#define GST_OPT_SSSP_DELTA 10      /* one-label phase in bands of label value: first band = value,
+                                      doubling up to the radius (hop-bounded: until every label
+                                      is final); 0 = unordered; -1 (default) = 16 */
 #define GST_OPT_RADIUS_PCT 12      /* one-label radius = upper bound * value / 100 + 2 (default 100);
                                       distances beyond it are clamped lower bounds */
 #define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */

@@ -179,6 +179,33 @@ def test_hop_band_schedules(gpu, div):
         _check_tree(g, grp, q, br.costs[i], br.tree(i), int(D[i]))
 
 
+@pytest.mark.parametrize("band", [0, 1, 16])
+def test_hop_one_label_bands(gpu, band):
+    # the one-label phase of hop-bounded queries (no radius) unordered and in
+    # doubling bands: labels must end exact either way, or the completion
+    # bounds would prune optimal trees
+    g, grp = workloads.small(160, 450, 8, seed_=59, whi=90, member_p=0.04)
+    G = _graph(g, grp)
+    G.set_option("sssp_delta", band)
+    rng = np.random.default_rng(61)
+    qs = [rng.choice(8, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 36)]
+    D = rng.integers(1, 6, 36).astype(np.int32)
+    _say(f"hop one-label bands={band}: batch of 36 hop queries n={g.n}")
+    br = G.solve_batch(qs, D=D)
+    for i, q in enumerate(qs):
+        ref = O.hop(g, grp, q, int(D[i]))
+        if ref["status"] == O.INFEASIBLE:
+            assert br.status[i] == STATUS_INFEASIBLE
+            continue
+        assert br.costs[i] == ref["cost"], (band, i, br.costs[i], ref["cost"])
+        _check_tree(g, grp, q, br.costs[i], br.tree(i), int(D[i]))
+    _say(f"hop one-label bands={band}: single query")
+    r = G.solve(qs[0], D=int(D[0]))
+    ref = O.hop(g, grp, qs[0], int(D[0]))
+    if ref["status"] == O.OK:
+        assert r.cost == ref["cost"]
+
+
 def test_list_overflow_reruns_exactly(gpu):
     # lists far too small for the batch: some list (near, far, one-label or DP)
     # overflows, the batch is re-run with full-size lists, results stay exact;
