@@ -114,6 +114,12 @@ def cpu_baselines(w, budget_s: float, gpu, D=None):
     return mt, one
 
 
+def _ablation(args) -> list:
+    """Ablation switches in force (the reference's ablation variants)."""
+    return [n for n, on in (("lane_appends", args.lane_appends),
+                            ("per_item_edges", args.per_item_edges)) if on]
+
+
 def _apply_options(G, args):
     if args.chunk:
         G.set_option("chunk", args.chunk)
@@ -129,6 +135,10 @@ def _apply_options(G, args):
         G.set_option("max_batch", args.max_batch)
     if args.sssp_slots >= 0:
         G.set_option("sssp_slots", args.sssp_slots)
+    if args.lane_appends:
+        G.set_option("lane_appends", 1)
+    if args.per_item_edges:
+        G.set_option("per_item_edges", 1)
 
 
 def _profiled_roofline(G, w, D, hop: bool):
@@ -233,7 +243,8 @@ def other_workload(args):
         "config": {"workload": w.name, "V": w.graph.n, "E_undirected": w.graph.m,
                    "groups": w.groups.count, "queries_per_step": len(w.queries),
                    "k": sorted({len(q) for q in w.queries}),
-                   "hop_bound": None if D is None else sorted({int(x) for x in D})},
+                   "hop_bound": None if D is None else sorted({int(x) for x in D}),
+                   "ablation": _ablation(args)},
         "phase_ms": st["phase_ms"], "single_query_qps": ns / el1, "single_sample": ns,
         "batched_over_single": qps / (ns / el1),
         "cost_mismatch_batched_vs_single": mism, "costs_stable_across_steps": stable,
@@ -261,6 +272,10 @@ def main():
     ap.add_argument("--dp-div", type=int, default=-1,
                     help="A* band divisor of the plain DP (0 = unordered, -1 = automatic)")
     ap.add_argument("--grid", type=int, default=0)
+    ap.add_argument("--lane-appends", action="store_true",
+                    help="ablation: per-lane global appends (no LDS aggregation)")
+    ap.add_argument("--per-item-edges", action="store_true",
+                    help="ablation: each lane walks its own item's edges (no balanced mapping)")
     ap.add_argument("--radius-pct", type=int, default=0)
     ap.add_argument("--max-batch", type=int, default=0,
                     help="cap on queries per lockstep batch (0 = as many as the HBM budget admits)")
@@ -375,7 +390,7 @@ def main():
         "data": "synthetic (seeded Chung-Lu graph, Zipf groups, uniform queries)",
         "config": {"workload": w.name, "V": w.graph.n, "E_undirected": w.graph.m,
                    "groups": w.groups.count, "queries_per_step": len(w.queries),
-                   "k": [3, 5, 7], "hop_bound": None},
+                   "k": [3, 5, 7], "hop_bound": None, "ablation": _ablation(args)},
         "phase_ms": st["phase_ms"],
         "single_query_qps": single_qps,
         "single_query_chunk": args.single_chunk,

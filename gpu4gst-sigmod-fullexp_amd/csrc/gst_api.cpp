@@ -111,6 +111,8 @@ struct gst_graph {
   int32_t max_batch = 0;
   int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
   int32_t list_permille = 0;  // worklist capacity override (0 = default sizing)
+  int32_t lane_appends = 0;    // ablation: per-lane global appends
+  int32_t per_item_edges = 0;  // ablation: per-item edge walks
   int32_t maxw = 1;
   // engine buffers (grow-only)
   DBuf T, Es, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
@@ -215,6 +217,8 @@ DevCtx make_ctx(gst_graph* g) {
   c.far_sel = g->iter.as<int32_t>() + 2;
   c.sssp_far = g->sssp_band_eff > 0 ? 1 : 0;
   c.band_key = g->iter.as<int32_t>() + 3;
+  c.lane_appends = g->lane_appends;
+  c.per_item_edges = g->per_item_edges;
   return c;
 }
 
@@ -823,6 +827,14 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
     case GST_OPT_SSSP_DELTA:
       if (v < -1 || v > INF / 2) return fail(GST_ERR_INVALID, "band width out of range");
       g->sssp_band = (int32_t)v;
+      break;
+    case GST_OPT_LANE_APPENDS:
+      if (v < 0 || v > 1) return fail(GST_ERR_INVALID, "lane appends is 0 or 1");
+      g->lane_appends = (int32_t)v;
+      break;
+    case GST_OPT_PER_ITEM_EDGES:
+      if (v < 0 || v > 1) return fail(GST_ERR_INVALID, "per-item edges is 0 or 1");
+      g->per_item_edges = (int32_t)v;
       break;
     case GST_OPT_LIST_PERMILLE:
       if (v < 0 || v > 1000) return fail(GST_ERR_INVALID, "list permille out of range");

@@ -142,6 +142,9 @@ struct DevCtx {
   // the near list has drained and the bands have grown.
   int32_t sssp_far;
   int32_t* band_key;
+  // ablation switches: per-lane global appends, per-item edge walks
+  int32_t lane_appends;
+  int32_t per_item_edges;
   uint64_t* far[2];
   uint32_t* far_cnt;             // [2]
   int32_t* far_sel;

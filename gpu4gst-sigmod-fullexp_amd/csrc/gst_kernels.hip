@@ -89,6 +89,17 @@ __device__ inline void wave_append(uint32_t cnt, uint64_t item, uint64_t* out, u
 #endif
 constexpr int WSTAGE = GST_WSTAGE;
 
+// Ablation (GST_OPT_LANE_APPENDS): the lane reserves its own range with one
+// global atomic, no aggregation. Called by the lanes that have items only.
+__device__ inline void lane_append(uint32_t cnt, uint64_t item, uint64_t* out, uint32_t* out_cnt,
+                                   uint32_t cap, unsigned long long* ctr) {
+  const uint32_t base = atomicAdd(out_cnt, cnt);
+  for (uint32_t i = 0; i < cnt; ++i) {
+    if (base + i < cap) out[base + i] = item | ((uint64_t)i << 36);
+    else atomicAdd(&ctr[C_OVERFLOW], 1ull);
+  }
+}
+
 __device__ inline void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -114,7 +125,11 @@ __device__ inline void stage_flush(uint64_t* stage, uint32_t& n, uint64_t* out,
 // wavefront must call this together.
 __device__ inline void stage_append(uint64_t* stage, uint32_t& n, uint32_t cnt, uint64_t item,
                                     uint64_t* out, uint32_t* out_cnt, uint32_t cap,
-                                    unsigned long long* ctr) {
+                                    unsigned long long* ctr, bool per_lane = false) {
+  if (per_lane) {  // wave-uniform
+    if (cnt) lane_append(cnt, item, out, out_cnt, cap, ctr);
+    return;
+  }
   const int lane = lane_id();
   uint32_t incl = cnt;
 #pragma unroll
@@ -142,7 +157,14 @@ __device__ inline void stage_append(uint64_t* stage, uint32_t& n, uint32_t cnt, 
 // accumulator so a fast wave's next call cannot meet a stale one.
 __device__ inline void block_append(const uint32_t* cnt, const uint64_t* items, uint64_t* out,
                                     uint32_t* out_cnt, uint32_t cap, unsigned long long* ctr,
-                                    uint32_t* s_acc, uint32_t* s_base, int round) {
+                                    uint32_t* s_acc, uint32_t* s_base, int round,
+                                    bool per_lane = false) {
+  if (per_lane) {  // uniform over the workgroup
+#pragma unroll
+    for (int q = 0; q < EPL; ++q)
+      if (cnt[q]) lane_append(cnt[q], items[q], out, out_cnt, cap, ctr);
+    return;
+  }
   const int lane = lane_id();
   uint32_t mine = 0;
 #pragma unroll
@@ -290,6 +312,20 @@ __device__ inline void block_scan(int32_t x, int32_t* s_off, int32_t* s_wsum) {
   s_off[tid] = prefix + incl - x;
   if (tid == BLOCK - 1) s_off[BLOCK] = prefix + incl;
   __syncthreads();
+}
+
+// Largest x over the workgroup; every thread must call it.
+__device__ inline int32_t block_max(int32_t x, int32_t* s_wsum) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) x = max(x, __shfl_down(x, off, WAVE));
+  __syncthreads();  // s_wsum may still be read by an earlier scan
+  if (lane_id() == 0) s_wsum[threadIdx.x / WAVE] = x;
+  __syncthreads();
+  int32_t m = s_wsum[0];
+#pragma unroll
+  for (int w = 1; w < BLOCK / WAVE; ++w) m = max(m, s_wsum[w]);
+  __syncthreads();  // before s_wsum is reused
+  return m;
 }
 
 __device__ inline int find_item(const int32_t* s_off, int32_t e) {
@@ -524,29 +560,39 @@ __global__ __launch_bounds__(BLOCK) GST_SSSP_ATTR void k_sssp_expand(DevCtx c, W
     }
     if (banded)
       stage_append(fstage, fst_n, far_want, far_item, c.far[far_sel], c.far_cnt + far_sel, c.cap,
-                   c.ctr);
+                   c.ctr, c.lane_appends != 0);
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
     // Each lane takes EPL edges per round (strided by the block size, so
     // the CSR reads stay coalesced) and issues all their loads before using
     // any: the random neighbour-row reads of one lane are then in flight
-    // together instead of one after another.
-    for (int32_t base = 0; base < total; base += BLOCK * EPL) {
+    // together instead of one after another. (Ablation: each lane walks its
+    // own item's edges, EPL per round, as many rounds as the largest degree.)
+    const bool per_item = c.per_item_edges != 0;
+    const int32_t span = per_item ? block_max(deg, s_wsum) * BLOCK : total;
+    for (int32_t base = 0; base < span; base += BLOCK * EPL) {
       int jq[EPL];
+      int32_t oq[EPL];  // edge offset inside the item's chunk
       uint32_t uq[EPL];
       int32_t wq[EPL];
       int4 aq[EPL], bq[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
-        const int32_t e = base + q * BLOCK + tid;
-        jq[q] = e < total ? find_item(s_off, e) : -1;
+        if (per_item) {
+          oq[q] = base / BLOCK + q;
+          jq[q] = oq[q] < deg ? tid : -1;
+        } else {
+          const int32_t e = base + q * BLOCK + tid;
+          jq[q] = e < total ? find_item(s_off, e) : -1;
+          oq[q] = jq[q] >= 0 ? e - s_off[jq[q]] : 0;
+        }
       }
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
         uq[q] = 0;
         wq[q] = 0;
         if (jq[q] >= 0) {
-          const int32_t ej = s_beg[jq[q]] + (base + q * BLOCK + tid - s_off[jq[q]]);
+          const int32_t ej = s_beg[jq[q]] + oq[q];
           uq[q] = (uint32_t)c.col[ej];
           wq[q] = c.wt[ej];
         }
@@ -613,7 +659,8 @@ __global__ __launch_bounds__(BLOCK) GST_SSSP_ATTR void k_sssp_expand(DevCtx c, W
         cntq[q] = cnt;
         itemq[q] = item;
       }
-      block_append(cntq, itemq, w.out, w.out_cnt, c.cap, c.ctr, s_acc, &s_abase, ++round);
+      block_append(cntq, itemq, w.out, w.out_cnt, c.cap, c.ctr, s_acc, &s_abase, ++round,
+                   c.lane_appends != 0);
     }
     __syncthreads();
   }
@@ -1173,6 +1220,7 @@ __global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w)
   uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
   uint32_t st_n = 0, fst_n = 0;
   const bool far_mode = c.far_mode != 0;
+  const bool lane_ap = c.lane_appends != 0;
   const int far_sel = far_mode ? *c.far_sel : 0;
   uint64_t* far_out = c.far[far_sel];
   uint32_t* far_out_cnt = c.far_cnt + far_sel;
@@ -1269,9 +1317,10 @@ __global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w)
       }
     }
 
-    stage_append(stage, st_n, carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr);
+    stage_append(stage, st_n, carry_cnt, citem, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
     if (far_mode)
-      stage_append(fstage, fst_n, far_cnt_l, far_item_l, far_out, far_out_cnt, c.cap, c.ctr);
+      stage_append(fstage, fst_n, far_cnt_l, far_item_l, far_out, far_out_cnt, c.cap, c.ctr,
+                   lane_ap);
 
     // ---- merges, balanced over the workgroup like the grow edges: item j
     // owns pairs s_off[j] .. s_off[j + 1] - 1; its pair r merges with subset
@@ -1411,32 +1460,43 @@ __global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w)
             }
           }
         }
-        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
-        if (far_mode) stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
+        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
+        if (far_mode)
+          stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr, lane_ap);
       }
     }
     __syncthreads();  // s_off is reused by the grow scan
 
     // ---- grow: the workgroup's edges, balanced by an LDS prefix scan
+    // (ablation: each lane walks its own item's edges, as in the SSSP kernel)
     block_scan(deg, s_off, s_wsum);
     const int32_t total = s_off[BLOCK];
+    const bool per_item = c.per_item_edges != 0;
+    const int32_t span = per_item ? block_max(deg, s_wsum) * BLOCK : total;
     // EPL edges per lane per round, all loads issued before any is used
-    for (int32_t base = 0; base < total; base += BLOCK * EPL) {
+    for (int32_t base = 0; base < span; base += BLOCK * EPL) {
       int jq[EPL];
+      int32_t oq[EPL];  // edge offset inside the item's chunk
       uint32_t uq[EPL];
       int32_t gq[EPL];
       int2 dq[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
-        const int32_t e = base + q * BLOCK + tid;
-        jq[q] = e < total ? find_item(s_off, e) : -1;
+        if (per_item) {
+          oq[q] = base / BLOCK + q;
+          jq[q] = oq[q] < deg ? tid : -1;
+        } else {
+          const int32_t e = base + q * BLOCK + tid;
+          jq[q] = e < total ? find_item(s_off, e) : -1;
+          oq[q] = jq[q] >= 0 ? e - s_off[jq[q]] : 0;
+        }
       }
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
         uq[q] = 0;
         gq[q] = INF;
         if (jq[q] >= 0) {
-          const int32_t ej = s_beg[jq[q]] + (base + q * BLOCK + tid - s_off[jq[q]]);
+          const int32_t ej = s_beg[jq[q]] + oq[q];
           uq[q] = (uint32_t)c.col[ej];
           gq[q] = s_cost[jq[q]] + c.wt[ej];
         }
@@ -1491,8 +1551,9 @@ __global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w)
             }
           }
         }
-        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr);
-        if (far_mode) stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
+        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
+        if (far_mode)
+          stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr, lane_ap);
       }
     }
     __syncthreads();

@@ -17,7 +17,7 @@ EXPORTS = ("gst_graph_create", "gst_graph_set_edge_ids", "gst_graph_set_weights"
 OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_INTERNAL = 0, 1, 2, 3, 4, 5
 OPT = dict(chunk=1, grid=2, iters_per_sync=3, hop_prune=4, profile=5, infeasible_cost=6,
            max_tree_edges=7, use_graphs=8, sssp_delta=10, dp_delta_div=11, radius_pct=12, max_batch=14,
-           sssp_slots=15, list_permille=16)
+           sssp_slots=15, list_permille=16, lane_appends=17, per_item_edges=18)
 
 
 class GstStats(C.Structure):

@@ -111,6 +111,15 @@ typedef struct gst_stats {
                                       the usual floors (0 = default: 4%, at least one item per
                                       vertex and query); a list that overflows makes the batch
                                       re-run with full-size lists (gst_stats.reruns) */
+/* Ablation switches (the reference's TrimCDP-WB / D-TrimCDP-WB ablation variants, README.md:
+ * 181-302); results are the same, only the speed changes: */
+#define GST_OPT_LANE_APPENDS 17    /* 1 = every lane appends its frontier items with its own
+                                      global atomic (no LDS aggregation; variants without the
+                                      shared-memory prefix scan); default 0 */
+#define GST_OPT_PER_ITEM_EDGES 18  /* 1 = each lane walks its own item's edges in order (per-thread
+                                      contiguous chunks, the "no_global_memory_coalescing"
+                                      variants) instead of the workgroup's edges balanced and
+                                      lane-strided by an LDS prefix scan; default 0 */
 
 /* Creates a graph on the current HIP device from a symmetric CSR:
  * beg_pos[V+1], csr[E], weight[E] with E = beg_pos[V]; every undirected edge

@@ -118,6 +118,27 @@ def main(n_plain, n_hop):
         if ref < O.INF() and int(br.costs[i]) != ref:
             fails.append(("banded-batch", i, q.tolist(), int(br.costs[i]), int(ref)))
     G.close()
+    # the ablation switches (per-lane appends, per-item edge walks, small
+    # chunks): same results, plain and hop-bounded
+    g, grp = workloads.small(70, 190, 8, seed_=37, whi=40, member_p=0.05)
+    G = GSTGraph.from_generated(g)
+    G.set_groups(grp)
+    G.set_option("lane_appends", 1)
+    G.set_option("per_item_edges", 1)
+    G.set_option("chunk", 64)
+    rng = np.random.default_rng(41)
+    qs = [rng.choice(8, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 7, 10)]
+    Ds = rng.integers(1, 5, 10).astype(np.int32)
+    bp = G.solve_batch(qs)
+    bh = G.solve_batch(qs, D=Ds)
+    for i, q in enumerate(qs):
+        for res, ref, what in ((bp, O.dpbf(g, grp, q), "ablation-plain"),
+                               (bh, O.hop(g, grp, q, int(Ds[i])), "ablation-hop")):
+            cases += 1
+            ok = ref["status"] == O.OK
+            if ok != (int(res.status[i]) == STATUS_OK) or (ok and int(res.costs[i]) != ref["cost"]):
+                fails.append((what, i, q.tolist(), int(res.costs[i]), int(ref["cost"])))
+    G.close()
     print(json.dumps({"cases": cases, "fails": fails}))
 
 

@@ -206,6 +206,31 @@ def test_hop_one_label_bands(gpu, band):
         assert r.cost == ref["cost"]
 
 
+@pytest.mark.parametrize("lane,per_item", [(1, 0), (0, 1), (1, 1)])
+def test_ablation_switches(gpu, lane, per_item):
+    # the reference's ablation variants as switches: per-lane appends and
+    # per-item edge walks change the schedule, never the result (a small
+    # chunk makes hub rows split into several items)
+    g, grp = workloads.small(180, 520, 8, seed_=67, whi=70, member_p=0.04)
+    G = _graph(g, grp)
+    G.set_option("lane_appends", lane)
+    G.set_option("per_item_edges", per_item)
+    G.set_option("chunk", 64)
+    rng = np.random.default_rng(71)
+    qs = [rng.choice(8, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 40)]
+    D = rng.integers(1, 5, 40).astype(np.int32)
+    _say(f"ablation lane={lane} per_item={per_item}: 40 plain + 40 hop queries n={g.n}")
+    bp = G.solve_batch(qs)
+    bh = G.solve_batch(qs, D=D)
+    for i, q in enumerate(qs):
+        for res, ref, Dq in ((bp, O.dpbf(g, grp, q), -1), (bh, O.hop(g, grp, q, int(D[i])), int(D[i]))):
+            if ref["status"] == O.INFEASIBLE:
+                assert res.status[i] == STATUS_INFEASIBLE
+                continue
+            assert res.costs[i] == ref["cost"], (lane, per_item, i, Dq, res.costs[i], ref["cost"])
+            _check_tree(g, grp, q, res.costs[i], res.tree(i), Dq)
+
+
 def test_list_overflow_reruns_exactly(gpu):
     # lists far too small for the batch: some list (near, far, one-label or DP)
     # overflows, the batch is re-run with full-size lists, results stay exact;
