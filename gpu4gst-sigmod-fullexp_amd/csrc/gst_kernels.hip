@@ -1099,15 +1099,15 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
         tile[lane * TS + (p - h)] = lb;
       }
       __syncthreads();
+      // one lane per (vertex, subset); a hop slot's lane writes the word for
+      // every depth d (rows d * 2^k + p), consecutive lanes consecutive rows
       const int sh = __builtin_ctz((unsigned)np);
-      const int per_v = nd << sh;
-      for (int x = threadIdx.x; x < nv * per_v; x += BLOCK) {
-        const int vl = nd == 1 ? x >> sh : x / per_v;
-        const int rem = x - vl * per_v;
-        const int pp = rem & (np - 1);
+      for (int x = threadIdx.x; x < (nv << sh); x += BLOCK) {
+        const int vl = x >> sh, pp = x & (np - 1);
         if (h + pp == 0) continue;  // p = 0 is no state
-        const int row = (rem >> sh) * S.nsub + h + pp;
-        T2[t_ix(S, row, v0 + vl)] = make_int2(INF, tile[vl * TS + pp]);
+        const int2 word = make_int2(INF, tile[vl * TS + pp]);
+        int64_t ix = t_ix(S, h + pp, v0 + vl);
+        for (int dd = 0; dd < nd; ++dd, ix += S.nsub) T2[ix] = word;
       }
       __syncthreads();
     }
