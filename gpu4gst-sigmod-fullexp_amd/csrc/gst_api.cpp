@@ -113,6 +113,7 @@ struct gst_graph {
   int32_t list_permille = 0;  // worklist capacity override (0 = default sizing)
   int32_t lane_appends = 0;    // ablation: per-lane global appends
   int32_t per_item_edges = 0;  // ablation: per-item edge walks
+  int64_t eff_key = -1;        // the effective settings baked into the captured graphs
   int32_t maxw = 1;
   // engine buffers (grow-only)
   DBuf T, Es, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
@@ -393,6 +394,14 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   // visits; hop-bounded queries, which have no radius, on the 978k-vertex
   // workload: 619 vs 720 ms, 8.5 G vs 15.2 G edge visits)
   g->sssp_band_eff = g->sssp_band >= 0 ? g->sssp_band : kAutoSsspBand;
+  {
+    // both values are kernel arguments inside the captured graphs
+    const int64_t key = ((int64_t)g->dp_div_eff << 32) | (uint32_t)g->sssp_band_eff;
+    if (key != g->eff_key) {
+      g->eff_key = key;
+      grew = true;
+    }
+  }
   const bool far_lists = g->dp_div_eff > 0 || g->sssp_band_eff > 0;
   if (far_lists &&
       ((uint64_t)cap * 8 > g->far0.bytes || (uint64_t)cap * 8 > g->far1.bytes)) {
