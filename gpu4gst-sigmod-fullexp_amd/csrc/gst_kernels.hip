@@ -1026,22 +1026,27 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 #ifndef GST_LB_TP
-#define GST_LB_TP 32
+#define GST_LB_TP 16
+#endif
+#ifndef GST_LB_VPL
+#define GST_LB_VPL 2
 #endif
 __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   // A workgroup stages its slot's route tables in LDS (entries for groups
   // 0..6 only: 49 pair routes + 7 row minima per subset) and then walks tiles
-  // of 64 consecutive vertices, subsets in passes of TP (a smaller tile than
+  // of 128 consecutive vertices, subsets in passes of TP (a smaller tile than
   // all 128 subsets at once lets four workgroups share a CU). Each wave
   // takes every fourth subset p, so U = full ^ p is the same in all its lanes
-  // (one lane per vertex) and every route read is an LDS broadcast. The loop
+  // (two vertices per lane, which each route read serves) and every route
+  // read is an LDS broadcast. The loop
   // over the first group i of a pair branches (wave-uniformly) on i in U; the
   // second group j runs without branches: k_route leaves the routes of groups
   // outside U (and of groups >= k) at INF, which can never be the minimum.
   // The bounds go to an LDS tile [vertex][p] (odd stride: conflict-free),
   // which the workgroup writes out as whole {INF, bound} state words,
   // contiguous in the vertex-major table.
-  constexpr int TV = 64;
+  constexpr int VPL = GST_LB_VPL;  // vertices per lane: each route read serves VPL of them
+  constexpr int TV = WAVE * VPL;
   constexpr int TP = GST_LB_TP;
   constexpr int TS = TP + 1;
   constexpr int RC = 57;  // compact route row: [i * 7 + j] for i, j < 7, then [49 + i]
@@ -1061,14 +1066,15 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   const int nd = nrows / S.nsub;  // hop: D + 1 copies of the subsets
   int2* T2 = reinterpret_cast<int2*>(c.T);
   for (int64_t v0 = (int64_t)blockIdx.x * TV; v0 < c.V; v0 += (int64_t)gridDim.x * TV) {
-    const int64_t v = v0 + lane;
-    int32_t cs[7];
-    {
+    int32_t cs[VPL][7];
+#pragma unroll
+    for (int r = 0; r < VPL; ++r) {
+      const int64_t v = v0 + lane + WAVE * r;
       const int64_t vr = v < c.V ? v : v0;
       const int4 a = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW]);
       const int4 b = *reinterpret_cast<const int4*>(&c.C[S.c_off + vr * CROW + 4]);
-      cs[0] = a.x; cs[1] = a.y; cs[2] = a.z; cs[3] = a.w;
-      cs[4] = b.x; cs[5] = b.y; cs[6] = b.z;  // word 7 is the one-label stamp
+      cs[r][0] = a.x; cs[r][1] = a.y; cs[r][2] = a.z; cs[r][3] = a.w;
+      cs[r][4] = b.x; cs[r][5] = b.y; cs[r][6] = b.z;  // word 7 is the one-label stamp
     }
     const int nv = (int)min<int64_t>(TV, c.V - v0);
     for (int h = 0; h < S.nsub; h += TP) {
@@ -1077,26 +1083,46 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
         if (p == 0) continue;  // p = 0 is no state
         const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
         const int32_t* R = s_r + U * RC;
-        int32_t one = 0, mn = INF;
+        int32_t one[VPL], mn[VPL], lb1[VPL], lb2[VPL];
 #pragma unroll
-        for (int i = 0; i < 7; ++i) {
-          const bool in = (U >> i) & 1u;
-          one = max(one, in ? cs[i] : 0);
-          mn = min(mn, in ? cs[i] : INF);
+        for (int r = 0; r < VPL; ++r) {
+          one[r] = 0;
+          mn[r] = INF;
+          lb1[r] = INF;
+          lb2[r] = 0;
+#pragma unroll
+          for (int i = 0; i < 7; ++i) {
+            const bool in = (U >> i) & 1u;
+            one[r] = max(one[r], in ? cs[r][i] : 0);
+            mn[r] = min(mn[r], in ? cs[r][i] : INF);
+          }
         }
-        int32_t lb1 = INF, lb2 = 0;
 #pragma unroll
         for (int i = 0; i < 7; ++i) {
           if (!((U >> i) & 1u)) continue;  // wave-uniform
           // (c_i + R_ij + c_j) / 2 is monotone in R_ij + c_j: take the minimum first
-          int32_t m = INF;
+          int32_t m[VPL];
 #pragma unroll
-          for (int j = 0; j < 7; ++j) m = min(m, R[i * 7 + j] + cs[j]);
-          lb1 = min(lb1, (cs[i] + m) >> 1);
-          lb2 = max(lb2, (cs[i] + R[49 + i] + mn) >> 1);
+          for (int r = 0; r < VPL; ++r) m[r] = INF;
+#pragma unroll
+          for (int j = 0; j < 7; ++j) {
+            const int32_t rij = R[i * 7 + j];
+#pragma unroll
+            for (int r = 0; r < VPL; ++r) m[r] = min(m[r], rij + cs[r][j]);
+          }
+          const int32_t rmin = R[49 + i];
+#pragma unroll
+          for (int r = 0; r < VPL; ++r) {
+            lb1[r] = min(lb1[r], (cs[r][i] + m[r]) >> 1);
+            lb2[r] = max(lb2[r], (cs[r][i] + rmin + mn[r]) >> 1);
+          }
         }
-        const int32_t lb = U == 0 ? 0 : (one >= INF ? one : max(one, max(lb1, lb2)));
-        tile[lane * TS + (p - h)] = lb;
+#pragma unroll
+        for (int r = 0; r < VPL; ++r) {
+          const int32_t lb =
+              U == 0 ? 0 : (one[r] >= INF ? one[r] : max(one[r], max(lb1[r], lb2[r])));
+          tile[(lane + WAVE * r) * TS + (p - h)] = lb;
+        }
       }
       __syncthreads();
       // one lane per (vertex, subset); a hop slot's lane writes the word for
@@ -1118,7 +1144,7 @@ void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) 
   if (nslots <= 0) return;  // must run for every batch: it initialises the state words
   (void)max_nsub;
   // enough workgroups to fill the device for one slot as for a thousand
-  const int tiles = (c.V + 63) / 64;
+  const int tiles = (c.V + WAVE * GST_LB_VPL - 1) / (WAVE * GST_LB_VPL);
   int per_slot = (8192 + nslots - 1) / nslots;
   if (per_slot < 4) per_slot = 4;
   dim3 grid(tiles < per_slot ? tiles : per_slot, nslots);
