@@ -294,8 +294,9 @@ def main():
                          "workloads (3 hop-bounded, 4 forward + backward, 5 R-MAT scale 22)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
-                                   "(queries shard across GPUs with no exchange, see DESIGN.md)"}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (queries are "
+                                   "independent: a batch shards across GPUs as one process "
+                                   "per GPU with no exchange)"}))
         sys.exit(2)
 
     import torch

@@ -476,9 +476,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   launch_clamp(c, nslots, g->stream);
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
   launch_route(c, nslots, g->stream);
-  int max_nsub = 0;
-  for (const SlotDesc& d : sd) max_nsub = std::max(max_nsub, d.nsub);
-  launch_lb_table(c, nslots, max_nsub, g->stream);
+  launch_lb_table(c, nslots, g->stream);
   launch_dp_seed(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(),
                  cnt, g->stream);
   launch_dp_sched_init(c, nslots, g->stream);

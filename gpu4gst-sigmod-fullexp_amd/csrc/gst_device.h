@@ -181,7 +181,7 @@ void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float
                      float* gw, int64_t nund, hipStream_t st);
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st);
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
-void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st);
+void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st);
 void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
 void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
                        uint32_t* near_cnt, hipStream_t st);

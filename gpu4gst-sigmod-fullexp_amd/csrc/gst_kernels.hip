@@ -36,13 +36,6 @@ __device__ inline int lane_id() { return __lane_id(); }
 #define GST_EPL 4
 #endif
 constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
-// occupancy hints of the two expand kernels (empty by default)
-#ifndef GST_SSSP_ATTR
-#define GST_SSSP_ATTR
-#endif
-#ifndef GST_DP_ATTR
-#define GST_DP_ATTR
-#endif
 #ifndef GST_MPL
 #define GST_MPL 2
 #endif
@@ -466,7 +459,7 @@ void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const i
 // single row access at u. Plain queries drop labels at or beyond the running
 // upper bound + 2 (see k_clamp).
 
-__global__ __launch_bounds__(BLOCK) GST_SSSP_ATTR void k_sssp_expand(DevCtx c, WL w) {
+__global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
@@ -1140,9 +1133,8 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   }
 }
 
-void launch_lb_table(const DevCtx& c, int nslots, int max_nsub, hipStream_t st) {
+void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st) {
   if (nslots <= 0) return;  // must run for every batch: it initialises the state words
-  (void)max_nsub;
   // enough workgroups to fill the device for one slot as for a thousand
   const int tiles = (c.V + WAVE * GST_LB_VPL - 1) / (WAVE * GST_LB_VPL);
   int per_slot = (8192 + nslots - 1) / nslots;
@@ -1227,7 +1219,7 @@ void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_
 //     (lb from hop-free distances is a valid bound for hop-bounded trees)
 
 template <bool HOP>
-__global__ __launch_bounds__(BLOCK) GST_DP_ATTR void k_dp_expand(DevCtx c, WL w) {
+__global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];

@@ -14,7 +14,7 @@
 // parameters) and fill the same outputs: real_cost, rt (seconds of the solve),
 // RAM (finite-state count stand-in: DP states expanded) and ret's counters.
 // A harness keeps its types and defines DP_GPU / DP_gpu as one-line forwards
-// (INTEGRATION.md). Unlike the GPU reference they also return the tree.
+// to these. Unlike the GPU reference they also return the tree.
 #ifndef GST_REFERENCE_ADAPTERS_HPP
 #define GST_REFERENCE_ADAPTERS_HPP
 
