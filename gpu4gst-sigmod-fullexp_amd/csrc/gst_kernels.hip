@@ -496,16 +496,21 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       int4 a = *reinterpret_cast<const int4*>(row);
       int4 b = *reinterpret_cast<const int4*>(row + 4);
       const uint32_t pend = pending_labels((uint32_t)b.w, (uint32_t)cur);
-      // Banded: relax only below the band end (eff); a continuation (stamp
-      // bit 7, set when the band grew) re-relaxes every label, but for a
-      // label already below the previous band end only past its old cut.
+      // Banded: relax only below the band end (eff); a continuation (flag
+      // 0x80 in the item's subset field, set when the band grew) re-relaxes
+      // every label, but for a label already below the previous band end only
+      // past its old cut.
       int32_t eff = radius, prev = 0;
       if (banded) {
         eff = min(radius, c.sched[s].thr);
         prev = c.sched[s].prev;
       }
-      const uint32_t changed = pend == 0xffu ? 0x7fu : (pend & 0x7fu);
-      const bool cont = pend != 0xffu && (pend & 0x80u);
+      const bool cont = (it_p(it) & 0x80u) != 0;
+      // a continuation's labels were all relaxed in earlier bands unless
+      // marked for this iteration (none are: the near list was empty when it
+      // was made)
+      const uint32_t changed =
+          cont ? ((b.w >> 8) == (int32_t)cur ? pend & 0x7fu : 0u) : (pend == 0xffu ? 0x7fu : pend & 0x7fu);
       int32_t cv[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
       int32_t lo = INF, done_lim = INF, lo_all = INF;
       bool beyond = false;  // a label waits for a later band
@@ -866,8 +871,8 @@ void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* ne
 // grows (doubling, never past the radius best * pct / 100 + 2, which the
 // labels only need to reach; a hop-bounded slot has no radius and grows until
 // no vertex waits) and the vertices waiting in the far list come back as
-// continuations (stamp bit 7); vertices of slots already at the radius are
-// dropped.
+// continuations (flag 0x80 in the item); vertices of slots already at the
+// radius are dropped.
 
 __global__ void k_band_start(DevCtx c) { *c.band_key = *c.iter + 1; }
 
@@ -895,23 +900,17 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* nea
   const int sel = *c.far_sel;
   const uint32_t n = min(c.far_cnt[sel], c.cap);
   const uint64_t* far = c.far[sel];
-  const uint32_t cur = (uint32_t)*c.iter;  // the iteration that will read the near list
   for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
     const uint32_t i = base + threadIdx.x;
     uint32_t want = 0;
     uint64_t item = 0;
     if (i < n) {
       const uint64_t it = far[i];
-      const uint32_t v = it_v(it), s = it_s(it);
-      const Sched& x = c.sched[s];
+      const Sched& x = c.sched[it_s(it)];
+      const uint32_t v = it_v(it);
       if (x.prev < x.thr) {  // the slot's band grew: v continues
-        // A vertex enters the far list at most once per band (the band key in
-        // stamp row 0) and no stamp changes while the near list is empty, so
-        // a plain store marks the continuation.
-        const SlotDesc S = c.slots[s];
-        c.C[S.c_off + (int64_t)v * CROW + CSTAMP] = (int32_t)((cur << 8) | 0x80u);
         want = nchunks(c, v);
-        item = it;
+        item = it | ((uint64_t)0x80u << 24);  // continuation flag in the subset field
       }
     }
     stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
