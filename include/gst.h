@@ -138,8 +138,10 @@ int gst_groups_set(gst_graph* g, const int64_t* group_offsets, const int32_t* gr
                    int32_t G);
 int gst_set_option(gst_graph* g, int32_t option, int64_t value);
 
-/* One query: k (1..GST_MAX_K) group ids; D < 0 = plain TrimCDP, D >= 0 =
- * hop-bounded D-TrimCDP. Writes the optimal cost, the tree as vertex pairs
+/* One query: k (1..GST_MAX_K) distinct group ids (a repeated id, k outside
+ * 1..GST_MAX_K, an id outside 0..G-1 or D > GST_MAX_D is GST_ERR_INVALID);
+ * D < 0 = plain TrimCDP, D >= 0 = hop-bounded D-TrimCDP. An empty group makes
+ * the query infeasible. Writes the optimal cost, the tree as vertex pairs
  * (out_edge_uv[2*i], out_edge_uv[2*i+1]) up to edge_cap edges, and the edge
  * count. Infeasible queries return GST_OK with *out_status = GST_Q_INFEASIBLE
  * and no edges. out_edge_uv / out_stats / out_status may be NULL. */

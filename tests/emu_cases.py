@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 sys.path.insert(0, os.path.join(ROOT, "gpu4gst-sigmod-fullexp_amd"))
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-from gpu4gst import GSTGraph, workloads, STATUS_OK, STATUS_INFEASIBLE  # noqa: E402
+from gpu4gst import GSTError, GSTGraph, gen, workloads, STATUS_OK, STATUS_INFEASIBLE  # noqa: E402
 import oracle as O  # noqa: E402
 
 
@@ -138,6 +138,42 @@ def main(n_plain, n_hop):
             ok = ref["status"] == O.OK
             if ok != (int(res.status[i]) == STATUS_OK) or (ok and int(res.costs[i]) != ref["cost"]):
                 fails.append((what, i, q.tolist(), int(res.costs[i]), int(ref["cost"])))
+    G.close()
+    # edge cases: an empty group, the largest hop bound, k = 7 hop-bounded, and
+    # queries the solver must refuse (a group named twice, k = 8, D = 16, an
+    # unknown group id, k = 0)
+    g, grp = workloads.small(30, 70, 6, seed_=43, whi=20, member_p=0.2)
+    lists = [grp.members(i).tolist() for i in range(grp.count)] + [[]]
+    grp = gen.groups_from_lists(lists)
+    empty = grp.count - 1
+    full = [i for i in range(empty) if len(lists[i])]
+    G = GSTGraph.from_generated(g)
+    G.set_groups(grp)
+    edge = [("empty-group", np.array([full[0], empty], np.int32), -1),
+            ("empty-group-hop", np.array([empty, full[1]], np.int32), 2),
+            ("max-hop-bound", np.array(full[:6], np.int32), 15),
+            ("k7-hop", np.array(full[:7], np.int32), 3)]
+    for what, q, D in edge:
+        cases += 1
+        r = G.solve(q, D=D)
+        ref = O.dpbf(g, grp, q) if D < 0 else O.hop(g, grp, q, D)
+        ok = ref["status"] == O.OK
+        if ok != (r.status == STATUS_OK) or (ok and r.cost != ref["cost"]):
+            fails.append((what, q.tolist(), int(r.cost), int(ref["cost"])))
+        elif ok and not check_tree(g, grp, q, r.cost, r.edges, D):
+            fails.append((what + "-tree", q.tolist(), int(r.cost), int(ref["cost"])))
+    for what, q, D in (("repeated-group", np.array([full[0], full[1], full[0]], np.int32), -1),
+                       ("k8", np.arange(8, dtype=np.int32), -1),
+                       ("hop-bound-16", np.array(full[:2], np.int32), 16),
+                       ("group-id-range", np.array([grp.count], np.int32), -1),
+                       ("k0", np.zeros(0, np.int32), -1)):
+        cases += 1
+        try:
+            G.solve(q, D=D)
+            fails.append(("accepted " + what, q.tolist(), 0, 0))
+        except GSTError as e:
+            if e.code != 1:  # GST_ERR_INVALID
+                fails.append(("error code " + what, q.tolist(), e.code, 1))
     G.close()
     print(json.dumps({"cases": cases, "fails": fails}))
 

@@ -31,7 +31,7 @@ def test_emulated_kernels_match_oracle():
                        env=env, capture_output=True, text=True, timeout=600)
     assert r.returncode == 0, r.stderr[-3000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
-    assert out["cases"] == 127
+    assert out["cases"] == 136
     assert out["fails"] == [], out["fails"]
 
 
