@@ -280,6 +280,48 @@ def test_infeasible_and_trivial(gpu):
     assert G.solve(np.array([0, 2], np.int32)).cost == 100000
 
 
+def test_edge_cases_and_refused_queries(gpu):
+    # an empty group (infeasible, plain and hop), the largest hop bound,
+    # k = 7 hop-bounded, a batch mixing them, and queries the solver refuses
+    from gpu4gst import GSTError
+    g, grp = workloads.small(40, 100, 8, seed_=83, whi=30, member_p=0.15)
+    lists = [grp.members(i).tolist() for i in range(grp.count)] + [[]]
+    grp = gen.groups_from_lists(lists)
+    empty = grp.count - 1
+    full = [i for i in range(empty) if len(lists[i])]
+    G = _graph(g, grp)
+    cases = [(np.array([full[0], empty], np.int32), -1),
+             (np.array([empty, full[1]], np.int32), 2),
+             (np.array(full[:6], np.int32), 15),
+             (np.array(full[:7], np.int32), 3),
+             (np.array(full[:7], np.int32), -1)]
+    for q, D in cases:
+        _say(f"edge case q={q.tolist()} D={D}")
+        r = G.solve(q, D=D)
+        ref = O.dpbf(g, grp, q) if D < 0 else O.hop(g, grp, q, D)
+        if ref["status"] == O.INFEASIBLE:
+            assert r.status == STATUS_INFEASIBLE
+            continue
+        assert r.cost == ref["cost"], (q.tolist(), D, r.cost, ref["cost"])
+        _check_tree(g, grp, q, r.cost, r.edges, D)
+    _say("edge cases as one hop batch")
+    hq = [q for q, D in cases if D >= 0]
+    hD = np.array([D for q, D in cases if D >= 0], np.int32)
+    b = G.solve_batch(hq, D=hD)
+    for i, q in enumerate(hq):
+        ref = O.hop(g, grp, q, int(hD[i]))
+        assert (b.status[i] == STATUS_OK) == (ref["status"] == O.OK)
+        if ref["status"] == O.OK:
+            assert b.costs[i] == ref["cost"]
+    for q, D in ((np.array([full[0], full[1], full[0]], np.int32), -1),
+                 (np.arange(8, dtype=np.int32), -1),
+                 (np.array(full[:2], np.int32), 16),
+                 (np.array([grp.count], np.int32), -1)):
+        _say(f"refused q={q.tolist()} D={D}")
+        with pytest.raises(GSTError):
+            G.solve(q, D=D)
+
+
 def test_rmat15_queries_vs_prunedpp(gpu):
     w = workloads.config(1, n_queries=24)
     G = _graph(w.graph, w.groups)
