@@ -322,6 +322,36 @@ def test_edge_cases_and_refused_queries(gpu):
             G.solve(q, D=D)
 
 
+def test_full_size_workload_properties(gpu):
+    # The 168k-vertex, 6.8M-edge Chung-Lu workload at full size, through
+    # properties that do not need the oracle's full solve: every tree is a
+    # valid group Steiner tree weighing exactly its cost, batched and single
+    # costs agree, a cost is never below the distance between two of its
+    # groups (the tree holds a path between them), and two k = 3 queries
+    # equal restated PrunedDP++ bit for bit.
+    w = workloads.config(2)
+    G = _graph(w.graph, w.groups)
+    qs = w.queries[:96]
+    _say(f"full-size workload {w.name} n={w.graph.n} m={w.graph.m}: batch of 96")
+    b = G.solve_batch(qs)
+    for i, q in enumerate(qs):
+        assert b.status[i] == STATUS_OK
+        _check_tree(w.graph, w.groups, q, b.costs[i], b.tree(i))
+    for i in range(0, 96, 24):
+        _say(f"full-size single query {i} k={len(qs[i])}")
+        assert G.solve(qs[i]).cost == b.costs[i]
+    for i in range(0, 96, 32):
+        q = qs[i]
+        dist = [O.sssp(w.graph, w.groups, int(gid))[0] for gid in q]
+        pair = max(int(dist[a][w.groups.members(int(q[c]))].min())
+                   for a in range(len(q)) for c in range(len(q)) if a != c)
+        assert b.costs[i] >= pair, (i, b.costs[i], pair)
+    k3 = [i for i, q in enumerate(qs) if len(q) == 3][:2]
+    for i in k3:
+        _say(f"full-size query {i} vs PrunedDP++")
+        assert b.costs[i] == O.prunedpp(w.graph, w.groups, qs[i])["cost"]
+
+
 def test_rmat15_queries_vs_prunedpp(gpu):
     w = workloads.config(1, n_queries=24)
     G = _graph(w.graph, w.groups)
