@@ -352,6 +352,29 @@ def test_full_size_workload_properties(gpu):
         assert b.costs[i] == O.prunedpp(w.graph, w.groups, qs[i])["cost"]
 
 
+def test_full_size_hop_workload_properties(gpu):
+    # The 978k-vertex hop-bounded workload at full size: trees valid within
+    # their hop bound and weighing their cost, batched = single, and costs
+    # monotone in the bound: plain optimum <= cost(D = 5) <= cost(4) <= cost(3).
+    w = workloads.config(3)
+    G = _graph(w.graph, w.groups)
+    qs = w.queries[:48]
+    D = np.asarray(w.D[:48], np.int32)
+    _say(f"full-size hop workload {w.name} n={w.graph.n} m={w.graph.m}: batch of 48")
+    b = G.solve_batch(qs, D=D)
+    for i, q in enumerate(qs):
+        if b.status[i] == STATUS_OK:
+            _check_tree(w.graph, w.groups, q, b.costs[i], b.tree(i), int(D[i]))
+    for i in range(0, 48, 16):
+        _say(f"full-size hop query {i}: single, D = 3, 4, 5 and plain")
+        assert G.solve(qs[i], D=int(D[i])).cost == b.costs[i]
+        c = [G.solve(qs[i], D=d) for d in (3, 4, 5)]
+        plain = G.solve(qs[i])
+        feas = [r.cost for r in c if r.status == STATUS_OK]
+        assert feas == sorted(feas, reverse=True)
+        assert all(plain.cost <= x for x in feas)
+
+
 def test_rmat15_queries_vs_prunedpp(gpu):
     w = workloads.config(1, n_queries=24)
     G = _graph(w.graph, w.groups)
