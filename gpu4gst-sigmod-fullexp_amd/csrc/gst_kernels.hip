@@ -228,6 +228,30 @@ __device__ inline int32_t row_cut(const int32_t* wt, int32_t lo, int32_t hi, int
   return lo;
 }
 
+// The same cut found by galloping from the row start, then a binary search
+// in the last gap: the one-label phase's band limits are small, so its cuts
+// lie near the start and the probes stay in the row's first cache lines.
+__device__ inline int32_t row_cut_near(const int32_t* wt, int32_t lo, int32_t hi,
+                                       int32_t limit) {
+  if (limit >= INF) return hi;
+  if (limit < 1 || lo >= hi) return lo;
+  for (int32_t step = 1;; step <<= 1) {
+    const int32_t probe = lo + step - 1;
+    if (probe >= hi) break;
+    if (wt[probe] > limit) {
+      hi = probe;
+      break;
+    }
+    lo = probe + 1;
+  }
+  while (lo < hi) {
+    const int32_t mid = (lo + hi) >> 1;
+    if (wt[mid] <= limit) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Lower bound of any tree that contains u and touches every group in U
 // (U != 0), from the one-label distances cs = c_i(u): the largest distance,
 // and the two tour bounds of PrunedDP++ (code/PrunedDP++/include/
@@ -535,8 +559,8 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       int32_t re = rb0;
       if (lo < INF) {
         // a relaxation must stay below the band end: weight <= eff - 1 - lo
-        const int32_t rb = done_lim < INF ? row_cut(c.wt, rb0, rend, done_lim) : rb0;
-        re = row_cut(c.wt, rb0, rend, eff < INF ? eff - 1 - lo : INF);
+        const int32_t rb = done_lim < INF ? row_cut_near(c.wt, rb0, rend, done_lim) : rb0;
+        re = row_cut_near(c.wt, rb0, rend, eff < INF ? eff - 1 - lo : INF);
         deg = max(0, re - rb);
         s_beg[tid] = rb;
         s_cv[tid][0] = make_int4(cv[0], cv[1], cv[2], cv[3]);
@@ -548,7 +572,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       // work left for a later band: a label not yet below the band end, or
       // edges past this band's cut that the final radius would still use
       if (banded && eff < radius && lo_all < INF) {
-        const int32_t cut_all = lo < INF ? re : row_cut(c.wt, rb0, rend, eff - 1 - lo_all);
+        const int32_t cut_all = lo < INF ? re : row_cut_near(c.wt, rb0, rend, eff - 1 - lo_all);
         const bool more = cut_all < rend && c.wt[cut_all] <= radius - 1 - lo_all;
         if ((beyond || more) && atomicMax(&c.E[e_ix(S, 0, v)], bkey) < bkey) {
           far_want = 1;
