@@ -219,7 +219,8 @@ __device__ inline int32_t sssp_radius(const DevCtx& c, int32_t best) {
 // `limit` (edges past it cannot lead to a useful relaxation).
 __device__ inline int32_t row_cut(const int32_t* wt, int32_t lo, int32_t hi, int32_t limit) {
   if (limit >= INF) return hi;
-  if (limit < 1) return lo;
+  if (limit < 1 || lo >= hi) return lo;
+  if (wt[hi - 1] <= limit) return hi;  // the whole row (common for the DP's wide limits)
   while (lo < hi) {
     const int32_t mid = (lo + hi) >> 1;
     if (wt[mid] <= limit) lo = mid + 1;
