@@ -49,14 +49,26 @@ def _traffic(workload: str) -> dict:
         return json.load(f).get("kernels", {})
 
 
-def _roof(kernel, bytes_alg, kernel_ms, launches, traffic):
+def _roof(kernel, bytes_alg, kernel_ms, launches, traffic, bytes_layout=None):
+    """Roofline of one kernel: algorithmic bytes per launch / average launch
+    time. For the DP expand kernel the bytes are 16 F + 12 Rg + 8 Rm + 12 U +
+    4 Lb (frontier states, grow and merge relaxations, improvements, one-label
+    terms of the bound tests); `achieved_layout` prices the same work with the
+    bytes this build's layout moves (8-byte state words, 64-bit items, 32-byte
+    one-label rows for upper bounds). `traffic_over_alg` = counter-measured HBM
+    bytes per launch / algorithmic bytes per launch."""
     achieved = bytes_alg / (kernel_ms / 1e3) / 1e9 if kernel_ms > 0 else 0.0
-    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic["bytes_per_launch"] if traffic else None,
-            "kernel": kernel, "kernel_ms_total": kernel_ms, "launches": launches,
-            "bytes_alg_per_launch": bytes_alg / launches,
-            "kernel_ms_per_launch": kernel_ms / launches}
+    per_launch = bytes_alg / launches
+    tr = traffic["bytes_per_launch"] if traffic else None
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": tr,
+           "traffic_over_alg": tr / per_launch if tr and per_launch else None,
+           "kernel": kernel, "kernel_ms_total": kernel_ms, "launches": launches,
+           "bytes_alg_per_launch": per_launch,
+           "kernel_ms_per_launch": kernel_ms / launches}
+    if bytes_layout is not None and kernel_ms > 0:
+        out["achieved_layout"] = bytes_layout / (kernel_ms / 1e3) / 1e9
+    return out
 
 
 def cpu_threads() -> int:
@@ -141,12 +153,29 @@ def _apply_options(G, args):
         G.set_option("per_item_edges", 1)
 
 
-def _profiled_roofline(G, w, D, hop: bool):
+def _write_trace(G, path):
+    """Per-launch trace of the profiled pass (phase, admission batch, items,
+    queries that expanded, queries in the batch, ms) as CSV."""
+    rows = G.trace()
+    with open(path, "w") as f:
+        f.write("launch,phase,batch,items,active_slots,slots,ms\n")
+        for i, r in enumerate(rows):
+            f.write(f"{i},{('one-label', 'dp', 'hop-dp')[r['phase']]},{r['batch']},{r['items']},"
+                    f"{r['active_slots']},{r['slots']},{r['ms']:.4f}\n")
+    log(f"trace: {len(rows)} launches -> {path}")
+
+
+def _profiled_roofline(G, w, D, hop: bool, trace_path=None):
     """Instrumented solve: every expand launch timed with HIP events on the
     solver's stream; returns (dominant, other) roofline dicts."""
     G.set_option("profile", 1)
+    if trace_path:
+        G.set_option("trace", 1)
     pr = G.solve_batch(w.queries, D=D)
     G.set_option("profile", 0)
+    if trace_path:
+        _write_trace(G, trace_path)
+        G.set_option("trace", 0)
     ps = pr.stats
     log(f"profiled: dp expand {ps['expand_ms']:.1f} ms / {ps['expand_launches']} launches, "
         f"one-label expand {ps['sssp_expand_ms']:.1f} ms")
@@ -158,7 +187,7 @@ def _profiled_roofline(G, w, D, hop: bool):
                       max(1, ps["sssp_expand_launches"]), traffic.get("k_sssp_expand"))
     kname = "k_dp_expand<true>" if hop else "k_dp_expand<false>"
     roof_dp = _roof(kname, ps["bytes_alg"], ps["expand_ms"], max(1, ps["expand_launches"]),
-                    traffic.get(kname))
+                    traffic.get(kname), ps["bytes_layout"])
     if ps["sssp_expand_ms"] >= ps["expand_ms"]:
         return roof_sssp, roof_dp
     return roof_dp, roof_sssp
@@ -231,8 +260,8 @@ def other_workload(args):
         single[i] = G.solve(w.queries[i], D=-1 if D is None else int(D[i])).cost
     el1 = time.perf_counter() - t1
     mism = sum(int(base.costs[i] != single[i]) for i in range(ns))
-    G.set_option("chunk", args.chunk or 4096)
-    dominant, other = _profiled_roofline(G, w, D, hop)
+    G.set_option("chunk", args.chunk)  # 0 = automatic again
+    dominant, other = _profiled_roofline(G, w, D, hop, args.trace)
     out = {
         "metric": ("GST queries/s, forward + backward (batched, exact cost)" if grad else
                    "GST queries/s (batched, exact cost" + (", hop-bounded)" if hop else ")")),
@@ -281,14 +310,16 @@ def main():
                     help="cap on queries per lockstep batch (0 = as many as the HBM budget admits)")
     ap.add_argument("--sssp-slots", type=int, default=-1,
                     help="queries per one-label pass inside a batch (0 = all at once)")
-    ap.add_argument("--single-chunk", type=int, default=256,
-                    help="virtual-split chunk for the single-query path (its best setting; "
-                         "0 = same as batched)")
+    ap.add_argument("--single-chunk", type=int, default=0,
+                    help="virtual-split chunk for the single-query path (0 = the library's "
+                         "automatic choice for one-query calls)")
     ap.add_argument("--profile-only", action="store_true",
                     help="only run the instrumented solve (the pass rocprofv3 summaries cover)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration) and exit")
     ap.add_argument("--single-dp-div", type=int, default=-1)
+    ap.add_argument("--trace", default="",
+                    help="write the per-launch trace of the profiled pass to this CSV file")
     ap.add_argument("--workload", type=int, default=2,
                     help="2 = the headline workload (default); 1, 3, 4, 5 = the other seeded "
                          "workloads (3 hop-bounded, 4 forward + backward, 5 R-MAT scale 22)")
@@ -370,10 +401,10 @@ def main():
     log(f"single: {single_qps:.1f} queries/s over {ns}; mismatches {mism_single}")
 
     if args.single_chunk:
-        G.set_option("chunk", args.chunk or 4096)
+        G.set_option("chunk", args.chunk)  # 0 = automatic again
     if args.single_dp_div >= 0:
         G.set_option("dp_delta_div", args.dp_div)  # -1 = automatic
-    dominant, other = _profiled_roofline(G, w, None, False)
+    dominant, other = _profiled_roofline(G, w, None, False, args.trace)
     relax = st["grow_relax"] + st["merge_relax"]
 
     out = {
@@ -394,7 +425,7 @@ def main():
                    "k": [3, 5, 7], "hop_bound": None, "ablation": _ablation(args)},
         "phase_ms": st["phase_ms"],
         "single_query_qps": single_qps,
-        "single_query_chunk": args.single_chunk,
+        "single_query_chunk": args.single_chunk or "automatic",
         "batched_over_single": qps / single_qps if single_qps else None,
         "single_sample": ns,
         "cost_mismatch_batched_vs_single": mism_single,

@@ -3,6 +3,8 @@
 libgst.so    HIP kernels + C ABI for gfx950 (hipcc), loaded by gpu4gst._lib
 libgstgen.so seeded workload generators (host only, g++)
 gst_run      the reference harnesses' command lines over libgst.so (host only, g++)
+_build/adapter_demo_gpu  tests/cpp/adapter_demo.cpp (reference-shaped C++ adapters)
+             linked against libgst.so, run by the GPU tests
 
 Every call recompiles from source; objects go to _build/ (git-ignored).
 """
@@ -19,6 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 ARCH = os.environ.get("GST_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 EMU_LIB = os.path.join(BUILD, "libgst_emu.so")
+ADAPTER_DEMO = os.path.join(BUILD, "adapter_demo_gpu")
 
 # Packed FP32 instructions are disabled for all device code.
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
@@ -57,6 +60,13 @@ def build(verbose: bool = False) -> None:
     _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC, os.path.join(CSRC, "gst_cli.cpp"),
           os.path.join(OUT, "libgst.so"), "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
     os.replace(tmp, os.path.join(OUT, "gst_run"))
+    # the reference-shaped C++ adapters (include/gst_reference_adapters.hpp) driven by
+    # tests/cpp/adapter_demo.cpp over the gfx950 library, for the GPU test suite
+    tmp = os.path.join(BUILD, "adapter_demo_gpu.tmp")
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INC,
+          os.path.join(os.path.dirname(HERE), "tests", "cpp", "adapter_demo.cpp"),
+          os.path.join(OUT, "libgst.so"), "-Wl,-rpath,$ORIGIN/../gpu4gst", "-o", tmp], verbose)
+    os.replace(tmp, ADAPTER_DEMO)
     tmp = os.path.join(OUT, "libgstgen.so.tmp")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall",
           os.path.join(CSRC, "gen.cpp"), "-o", tmp], verbose)

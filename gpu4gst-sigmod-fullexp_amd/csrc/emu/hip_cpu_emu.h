@@ -388,6 +388,14 @@ inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) {
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t) {
   return hipMemcpy(d, s, n, k);
 }
+inline hipError_t hipMemcpy2D(void* d, size_t dpitch, const void* s, size_t spitch, size_t width,
+                              size_t height, hipMemcpyKind) {
+  if (width > dpitch || width > spitch) return hipErrorInvalidValue;
+  for (size_t r = 0; r < height; ++r)
+    std::memcpy(static_cast<char*>(d) + r * dpitch, static_cast<const char*>(s) + r * spitch,
+                width);
+  return hipSuccess;
+}
 inline hipError_t hipMemset(void* d, int v, size_t n) {
   std::memset(d, v, n);
   return hipSuccess;

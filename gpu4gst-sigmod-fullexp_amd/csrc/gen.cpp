@@ -213,6 +213,34 @@ void gg_keep_lcc(gg_graph* g) {
   g->n = nn;
 }
 
+int64_t gg_lcc_vertices(const gg_graph* g, int32_t* out) {
+  std::vector<int64_t> parent(g->n);
+  std::iota(parent.begin(), parent.end(), 0);
+  auto find = [&](int64_t x) {
+    while (parent[x] != x) {
+      parent[x] = parent[parent[x]];
+      x = parent[x];
+    }
+    return x;
+  };
+  for (size_t i = 0; i < g->src.size(); ++i) {
+    int64_t a = find(g->src[i]), b = find(g->dst[i]);
+    if (a != b) parent[std::max(a, b)] = std::min(a, b);
+  }
+  std::vector<int64_t> size(g->n, 0);
+  for (int64_t v = 0; v < g->n; ++v) size[find(v)]++;
+  int64_t best = 0;
+  for (int64_t v = 1; v < g->n; ++v)
+    if (size[v] > size[best]) best = v;
+  int64_t k = 0;
+  for (int64_t v = 0; v < g->n; ++v)
+    if (find(v) == best) {
+      if (out) out[k] = (int32_t)v;
+      ++k;
+    }
+  return k;
+}
+
 void gg_set_weights_uniform(gg_graph* g, int32_t lo, int32_t hi, uint64_t seed) {
   Rng rng(seed);
   g->w.resize(g->src.size());
@@ -309,8 +337,17 @@ gg_groups* gg_groups_poisson(int64_t n, int32_t G, double lambda, uint64_t seed)
 
 gg_groups* gg_groups_zipf(int64_t n, int32_t G, double s, int32_t smin,
                           int32_t smax, uint64_t seed) {
-  if (n < 1 || G < 1 || smin < 1 || smax < smin) return nullptr;
-  if (smax > n) smax = (int32_t)n;
+  return gg_groups_zipf_pool(n, nullptr, n, G, s, smin, smax, seed);
+}
+
+gg_groups* gg_groups_zipf_pool(int64_t n, const int32_t* pool, int64_t npool, int32_t G,
+                               double s, int32_t smin, int32_t smax, uint64_t seed) {
+  if (n < 1 || npool < 1 || G < 1 || smin < 1 || smax < smin) return nullptr;
+  if (pool)
+    for (int64_t i = 0; i < npool; ++i)
+      if (pool[i] < 0 || pool[i] >= n) return nullptr;
+  const int64_t np = pool ? npool : n;
+  if (smax > np) smax = (int32_t)np;
   if (smin > smax) smin = smax;
   Rng rng(seed);
   // truncated Zipf over sizes 1..smax, then clipped to [smin, smax]
@@ -329,10 +366,10 @@ gg_groups* gg_groups_zipf(int64_t n, int32_t G, double s, int32_t smin,
     double r = rng.unit() * acc;
     int32_t size = (int32_t)(std::upper_bound(cdf.begin(), cdf.end(), r) - cdf.begin()) + 1;
     size = std::max(smin, std::min(smax, size));
-    // Floyd's algorithm: `size` distinct uniform vertices
+    // Floyd's algorithm: `size` distinct uniform positions of the pool
     picked.clear();
     EdgeSet seen(size + 1);
-    for (int64_t j = n - size; j < n; ++j) {
+    for (int64_t j = np - size; j < np; ++j) {
       int64_t t = (int64_t)rng.below((uint64_t)j + 1);
       if (seen.insert((uint64_t)t)) picked.push_back((int32_t)t);
       else {
@@ -340,6 +377,8 @@ gg_groups* gg_groups_zipf(int64_t n, int32_t G, double s, int32_t smin,
         picked.push_back((int32_t)j);
       }
     }
+    if (pool)
+      for (auto& x : picked) x = pool[x];
     std::sort(picked.begin(), picked.end());
     out->off[gi + 1] = out->off[gi] + size;
     mem.insert(mem.end(), picked.begin(), picked.end());

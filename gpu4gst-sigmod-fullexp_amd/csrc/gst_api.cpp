@@ -76,6 +76,19 @@ struct QueryIn {
   const int32_t* groups;
 };
 
+// Expand-kernel settings chosen per size class of a call: -1 / 0 = the rule.
+struct Tune {
+  int32_t chunk_shift = -1;  // log2 of the virtual-split chunk
+  int32_t grid = 0;          // workgroups of the expand kernels
+  int32_t dp_div = -1;       // first DP band = upper bound / dp_div (0 = unordered)
+  int32_t sssp_band = -1;    // first one-label band width (0 = unordered)
+};
+
+// Size classes of a call: one query, a few (2..31), many (32+: the banded
+// DP's regime).
+constexpr int kSizeClasses = 3;
+int size_class(int64_t nq) { return nq <= 1 ? 0 : (nq < 32 ? 1 : 2); }
+
 }  // namespace
 
 struct gst_graph {
@@ -95,8 +108,25 @@ struct gst_graph {
   std::vector<int64_t> h_goff;
   DBuf goff, gmem;
   // options
-  int32_t chunk_shift = 12;
-  int32_t grid = 0;
+  // Settings of the expand kernels. `*_opt` hold the caller's explicit
+  // choices (-1 / 0 = not set); the values in force for a call come from
+  // them, else from the autotuner's measured choice for the call's size
+  // class, else from the size-class model (select_settings).
+  int32_t chunk_opt = -1;   // log2 of the virtual-split chunk
+  int32_t chunk_shift = 12; // in force
+  int32_t grid = 0;         // explicit workgroup count of the expand kernels (0 = not set)
+  int32_t grid_eff = 0;     // in force
+  Tune tuned[kSizeClasses];
+  bool tuned_ok[kSizeClasses] = {false, false, false};
+  int32_t dp_div_call = -1, sssp_band_call = -1;  // per call: explicit or tuned (-1 = rule)
+  // backpointers: 1 = stored, 0 = recomputed from the cost table, -1 = stored
+  // when the whole call fits one admission batch with them (memory not tight)
+  int32_t bp_mode = -1;
+  bool bp_call = false;  // in force for the call being solved
+  // per-iteration trace of the profiled path (GST_OPT_TRACE with GST_OPT_PROFILE)
+  int32_t trace = 0;
+  int32_t batch_no = 0;
+  std::vector<gst_trace_row> trace_rows;
   int32_t iters_per_sync = 32;
   int32_t hop_prune = 1;
   int32_t profile = 0;
@@ -116,7 +146,7 @@ struct gst_graph {
   int64_t eff_key = -1;        // the effective settings baked into the captured graphs
   int32_t maxw = 1;
   // engine buffers (grow-only)
-  DBuf T, Es, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
+  DBuf T, Es, BPs, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
   DBuf out_uv, out_eid, out_n, out_err;
   uint32_t* h_cnt = nullptr;  // pinned
   int32_t h_nslots[1] = {0};  // slots of the batch being solved
@@ -152,14 +182,15 @@ struct gst_graph {
   // they count as free when the next batch is admitted
   size_t held_bytes() const {
     size_t s = 0;
-    for (const DBuf* b : {&T, &Es, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
+    for (const DBuf* b : {&T, &Es, &BPs, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
                           &out_eid})
       s += b->bytes;
     return s;
   }
   void release_batch_buffers() {
     drop_graphs();
-    for (DBuf* b : {&T, &Es, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv, &out_eid})
+    for (DBuf* b : {&T, &Es, &BPs, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
+                    &out_eid})
       b->release();
     wl_cap = 0;
     ++gen;
@@ -174,6 +205,9 @@ int32_t rows_of(int32_t k, int32_t D) { return (D < 0 ? 1 : D + 1) << k; }
 // power-law graphs stay near 1%); a batch that overflows is re-run with
 // worklists as large as the state count, which can never overflow.
 constexpr double kWorklistFraction = 0.04;
+// Largest worklist (items): positions must stay below 2^32 with room for the
+// appends of one round past the end.
+constexpr int64_t kListClamp = (int64_t)UINT32_MAX - 65536 - 4096;
 constexpr int kAutoBandSlots = 32;  // automatic DP bands from this many plain queries per batch
 constexpr int kAutoBandDiv = 256;   // first band width = upper bound / 256 (>= 1), doubling
 constexpr int kAutoSsspBand = 16;   // first one-label band: labels below 16, doubling per band
@@ -185,6 +219,9 @@ int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
 }
 
 int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
+
+// Device bytes of one slot's stored backpointers (one int32 per state).
+int64_t bp_bytes(const gst_graph* g, int32_t k, int32_t D) { return 4 * rows_of(k, D) * g->Vp; }
 
 DevCtx make_ctx(gst_graph* g) {
   DevCtx c;
@@ -220,10 +257,50 @@ DevCtx make_ctx(gst_graph* g) {
   c.band_key = g->iter.as<int32_t>() + 3;
   c.lane_appends = g->lane_appends;
   c.per_item_edges = g->per_item_edges;
+  c.BP = g->bp_call ? g->BPs.as<int32_t>() : nullptr;
+  c.trace = g->profile && g->trace ? 1 : 0;
   return c;
 }
 
-int expand_grid(const gst_graph* g) { return g->grid > 0 ? g->grid : g->cus * 8; }
+int expand_grid(const gst_graph* g) { return g->grid_eff > 0 ? g->grid_eff : g->cus * 8; }
+
+// Size-class model of the expand settings, from sweeps on the 168k-vertex /
+// 6.8M-edge power-law workload (chunk 128..8192, grid 2..16 x CUs): a lone
+// query has little parallelism besides its hub rows, so they are split into
+// 256-edge chunks (260 queries/s single vs 190 at 4096); a batch of many
+// queries fills the device with items and coarse chunks keep the per-item
+// overhead low (4096: 2,190 queries/s vs 1,960 at 256); the DP and one-label
+// band rules stay with solve_lockstep's defaults.
+Tune model_settings(const gst_graph* g, int cls) {
+  Tune t;
+  t.chunk_shift = cls == 0 ? 8 : (cls == 1 ? 10 : 12);
+  t.grid = g->cus * 8;
+  return t;
+}
+
+// Settings in force for a call of nq queries: the caller's explicit options,
+// else the autotuner's choice for the size class, else the model.
+void select_settings(gst_graph* g, int64_t nq) {
+  const int cls = size_class(nq);
+  Tune t = model_settings(g, cls);
+  if (g->tuned_ok[cls]) {
+    const Tune& m = g->tuned[cls];
+    if (m.chunk_shift >= 0) t.chunk_shift = m.chunk_shift;
+    if (m.grid > 0) t.grid = m.grid;
+    t.dp_div = m.dp_div;
+    t.sssp_band = m.sssp_band;
+  }
+  const int32_t cs = g->chunk_opt >= 0 ? g->chunk_opt : t.chunk_shift;
+  const int32_t gr = g->grid > 0 ? g->grid : t.grid;
+  g->dp_div_call = g->dp_div >= 0 ? g->dp_div : t.dp_div;
+  g->sssp_band_call = g->sssp_band >= 0 ? g->sssp_band : t.sssp_band;
+  if (cs != g->chunk_shift || gr != g->grid_eff) {  // both are baked into captured graphs
+    g->chunk_shift = cs;
+    g->grid_eff = gr;
+    ++g->gen;
+    g->drop_graphs();
+  }
+}
 
 // One frontier iteration pair: list 0 -> list 1, list 1 -> list 0.
 void launch_pair(gst_graph* g, const DevCtx& c, int mode) {
@@ -253,6 +330,9 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
       WL a{g->wl0.as<uint64_t>(), cnt, g->wl1.as<uint64_t>(), cnt + 1};
       WL b{g->wl1.as<uint64_t>(), cnt + 1, g->wl0.as<uint64_t>(), cnt};
       for (int h = 0; h < 2; ++h) {
+        uint32_t items = 0;
+        if (g->trace)
+          HIP_TRY(hipMemcpy(&items, cnt + h, sizeof(items), hipMemcpyDeviceToHost));
         HIP_TRY(hipEventRecord(g->ev0, g->stream));
         if (mode == 0) launch_sssp_expand(c, h ? b : a, grid, g->stream);
         else launch_dp_expand(c, h ? b : a, grid, mode == 2, g->stream);
@@ -263,6 +343,20 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
         HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
         if (ms_acc) *ms_acc += ms;
         if (launches) ++*launches;
+        if (g->trace) {
+          uint64_t active = 0;
+          HIP_TRY(hipStreamSynchronize(g->stream));
+          HIP_TRY(hipMemcpy(&active, g->ctr.as<uint64_t>() + C_ACTIVE, sizeof(active),
+                            hipMemcpyDeviceToHost));
+          gst_trace_row r;
+          r.phase = mode;
+          r.batch = g->batch_no;
+          r.items = items;
+          r.active_slots = (int32_t)active;
+          r.slots = *g->h_nslots;
+          r.ms = ms;
+          g->trace_rows.push_back(r);
+        }
       }
       pairs = 1;
     } else if (g->use_graphs) {
@@ -362,13 +456,16 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   }
   int64_t cap = std::max((int64_t)((double)items * wl_frac), sssp_items) + 4096;
   if (g->list_permille > 0 && wl_frac < 1.0) cap = items * g->list_permille / 1000 + 64;
-  if (cap > (int64_t)UINT32_MAX - 65536) cap = (int64_t)UINT32_MAX - 65536;
+  if (cap > kListClamp + 4096) cap = kListClamp + 4096;
   bool grew = false;
   HIP_TRY(g->T.ensure((size_t)t_tot * 8, &grew));
   bool e_grew = false;
   HIP_TRY(g->Es.ensure((size_t)e_tot * 4, &e_grew));
   if (e_grew) HIP_TRY(hipMemsetAsync(g->Es.p, 0, g->Es.bytes, g->stream));
   grew |= e_grew;
+  // backpointer words need no initialisation: every state the tree recovery
+  // visits was written in this batch, and each word is checked before use
+  if (g->bp_call) HIP_TRY(g->BPs.ensure((size_t)e_tot * 4, &grew));
   HIP_TRY(g->C.ensure((size_t)nslots * g->Vp * CROW * 4, &grew));
   HIP_TRY(g->GMs.ensure((size_t)nslots * g->Vp * 4, &grew));
   HIP_TRY(g->R.ensure((size_t)nslots * RTABLE * 4, &grew));
@@ -385,18 +482,20 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   // a lone query runs unordered (measured on the 168k-vertex workload with the
   // unordered one-label phase: 1,500 vs 1,360 queries/s batched, 188 vs 232
   // single; divisor 256 vs 64 with the banded one-label phase: 1,981 vs 1,915)
-  g->dp_div_eff = g->dp_div >= 0 ? g->dp_div
-                                 : (nslots >= kAutoBandSlots && qs[0].D < 0 ? kAutoBandDiv : 0);
+  g->dp_div_eff = g->dp_div_call >= 0
+                      ? g->dp_div_call
+                      : (nslots >= kAutoBandSlots && qs[0].D < 0 ? kAutoBandDiv : 0);
   // the one-label phase in bands of label value (delta stepping with near/far
   // lists), so that labels past the final radius are never computed and
   // label-correcting churn collapses (measured on the 168k-vertex workload:
   // 208 vs 326 ms of one-label time per batch of 1,024, 3.3 G vs 7.4 G edge
   // visits; hop-bounded queries, which have no radius, on the 978k-vertex
   // workload: 619 vs 720 ms, 8.5 G vs 15.2 G edge visits)
-  g->sssp_band_eff = g->sssp_band >= 0 ? g->sssp_band : kAutoSsspBand;
+  g->sssp_band_eff = g->sssp_band_call >= 0 ? g->sssp_band_call : kAutoSsspBand;
   {
-    // both values are kernel arguments inside the captured graphs
-    const int64_t key = ((int64_t)g->dp_div_eff << 32) | (uint32_t)g->sssp_band_eff;
+    // these values are kernel arguments inside the captured graphs
+    const int64_t key = ((int64_t)g->dp_div_eff << 32) | ((uint32_t)g->sssp_band_eff << 1) |
+                        (g->bp_call ? 1 : 0);
     if (key != g->eff_key) {
       g->eff_key = key;
       grew = true;
@@ -445,6 +544,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     g->h_nslots[0] = nslots;
     HIP_TRY(hipStreamSynchronize(g->stream));
   }
+  ++g->batch_no;
   HIP_TRY(hipEventRecord(g->ph[0], g->stream));
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
   launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
@@ -488,38 +588,116 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   if (rc) return rc;
   HIP_TRY(hipEventRecord(g->ph[3], g->stream));
   launch_backtrack(c, nslots, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
-                   g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), g->stream);
+                   g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), nullptr, g->stream);
   HIP_TRY(hipGetLastError());
   std::vector<uint64_t> ctr(C_NUM), bw(nslots);
   std::vector<int32_t> on(nslots), oerr(nslots);
+  std::vector<Sched> hsched(nslots);
   HIP_TRY(hipMemcpyAsync(ctr.data(), g->ctr.p, C_NUM * 8, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipMemcpyAsync(bw.data(), g->bestw.p, nslots * 8, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipMemcpyAsync(on.data(), g->out_n.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipMemcpyAsync(oerr.data(), g->out_err.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(hsched.data(), g->sched.p, nslots * sizeof(Sched), hipMemcpyDeviceToHost,
+                         g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (ctr[C_OVERFLOW]) return 100;  // worklist too small: caller retries with full capacity
   float ph_ms[3] = {0, 0, 0};
   for (int i = 0; i < 3; ++i) HIP_TRY(hipEventElapsedTime(&ph_ms[i], g->ph[i], g->ph[i + 1]));
-  std::vector<int32_t> huv((size_t)nslots * maxe * 2), hid((size_t)nslots * maxe);
-  HIP_TRY(hipMemcpy(huv.data(), g->out_uv.p, huv.size() * 4, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(hid.data(), g->out_eid.p, hid.size() * 4, hipMemcpyDeviceToHost));
+  // only the filled prefix of every slot's edge row comes back
+  int32_t max_on = 0;
+  for (int s = 0; s < nslots; ++s)
+    if (oerr[s] == 0) max_on = std::max(max_on, std::min(on[s], maxe));
+  std::vector<int32_t> huv((size_t)nslots * std::max(max_on, 1) * 2),
+      hid((size_t)nslots * std::max(max_on, 1));
+  if (max_on > 0) {
+    HIP_TRY(hipMemcpy2D(huv.data(), (size_t)max_on * 8, g->out_uv.p, (size_t)maxe * 8,
+                        (size_t)max_on * 8, nslots, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy2D(hid.data(), (size_t)max_on * 4, g->out_eid.p, (size_t)maxe * 4,
+                        (size_t)max_on * 4, nslots, hipMemcpyDeviceToHost));
+  }
+  // Trees longer than the per-slot buffer (code 2) are recovered again with
+  // room for min(cost, 4V) edges: every recovered edge weighs >= 1 and their
+  // total is at most the cost.
+  std::vector<int32_t> redo;
+  for (int s = 0; s < nslots; ++s)
+    if (oerr[s] == 2) redo.push_back(s);
+  std::map<int, std::pair<std::vector<int32_t>, std::vector<int32_t>>> long_trees;
+  if (!redo.empty()) {
+    int64_t cap2 = 1;
+    for (int s : redo)
+      cap2 = std::max<int64_t>(cap2, std::min<int64_t>((int64_t)(bw[s] >> 32) + 1,
+                                                        4 * (int64_t)g->V + 64));
+    const int nr = (int)redo.size();
+    DBuf uv2, id2, n2, e2, sel;
+    auto release = [&]() {
+      for (DBuf* b : {&uv2, &id2, &n2, &e2, &sel}) b->release();
+    };
+    hipError_t he = uv2.ensure((size_t)nr * cap2 * 8);
+    if (he == hipSuccess) he = id2.ensure((size_t)nr * cap2 * 4);
+    if (he == hipSuccess) he = n2.ensure((size_t)nr * 4);
+    if (he == hipSuccess) he = e2.ensure((size_t)nr * 4);
+    if (he == hipSuccess) he = sel.ensure((size_t)nr * 4);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(sel.p, redo.data(), nr * 4, hipMemcpyHostToDevice, g->stream);
+    if (he == hipSuccess) {
+      launch_backtrack(c, nr, (int)cap2, uv2.as<int32_t>(), id2.as<int32_t>(), n2.as<int32_t>(),
+                       e2.as<int32_t>(), sel.as<int32_t>(), g->stream);
+      he = hipGetLastError();
+    }
+    std::vector<int32_t> n2h(nr), e2h(nr), uvh, idh;
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(n2h.data(), n2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(e2h.data(), e2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(g->stream);
+    for (int r = 0; r < nr && he == hipSuccess; ++r) {
+      const int s = redo[r];
+      oerr[s] = e2h[r];
+      if (e2h[r] != 0) continue;
+      uvh.resize((size_t)n2h[r] * 2);
+      idh.resize((size_t)n2h[r]);
+      he = hipMemcpy(uvh.data(), uv2.as<int32_t>() + (size_t)r * cap2 * 2, uvh.size() * 4,
+                     hipMemcpyDeviceToHost);
+      if (he == hipSuccess)
+        he = hipMemcpy(idh.data(), id2.as<int32_t>() + (size_t)r * cap2, idh.size() * 4,
+                       hipMemcpyDeviceToHost);
+      long_trees[s] = {uvh, idh};
+    }
+    release();
+    HIP_TRY(he);
+  }
   for (int s = 0; s < nslots; ++s) {
     const int qi = qs[s].idx;
     const int32_t cost = (int32_t)(bw[s] >> 32);
+    if (oerr[s] == 2) {  // still longer than the retry's buffer: cost only
+      out.cost[qi] = cost;
+      out.status[qi] = GST_Q_NO_TREE;
+      out.uv[qi].clear();
+      out.ids[qi].clear();
+      continue;
+    }
     if (oerr[s] != 0)
       return fail(GST_ERR_INTERNAL, "tree recovery failed for query " + std::to_string(qi) +
                                         " (code " + std::to_string(oerr[s]) + ")");
     if (cost >= INF) {
       out.cost[qi] = g->infeasible_cost;
-      out.status[qi] = GST_Q_INFEASIBLE;
+      out.status[qi] = hsched[s].sat ? GST_Q_RANGE : GST_Q_INFEASIBLE;
       continue;
     }
     out.cost[qi] = cost;
     out.status[qi] = GST_Q_OK;
     std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> es;
-    for (int i = 0; i < on[s]; ++i) {
-      int32_t a = huv[((size_t)s * maxe + i) * 2], b = huv[((size_t)s * maxe + i) * 2 + 1];
-      es.push_back({{std::min(a, b), std::max(a, b)}, hid[(size_t)s * maxe + i]});
+    auto lt = long_trees.find(s);
+    if (lt != long_trees.end()) {
+      const auto& uvs = lt->second.first;
+      const auto& ids = lt->second.second;
+      for (size_t i = 0; i < ids.size(); ++i)
+        es.push_back({{std::min(uvs[2 * i], uvs[2 * i + 1]), std::max(uvs[2 * i], uvs[2 * i + 1])},
+                      ids[i]});
+    }
+    for (int i = 0; lt == long_trees.end() && i < on[s]; ++i) {
+      int32_t a = huv[((size_t)s * max_on + i) * 2], b = huv[((size_t)s * max_on + i) * 2 + 1];
+      es.push_back({{std::min(a, b), std::max(a, b)}, hid[(size_t)s * max_on + i]});
     }
     std::sort(es.begin(), es.end());
     es.erase(std::unique(es.begin(), es.end(),
@@ -546,6 +724,9 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     st->merge_relax += (int64_t)ctr[C_RM];
     st->improvements += (int64_t)ctr[C_U];
     st->lb_terms += (int64_t)ctr[C_LB];
+    st->ub_rows += (int64_t)ctr[C_UBROW];
+    st->bp_fallbacks += (int64_t)ctr[C_BP_MISS];
+    st->backpointers_stored = g->bp_call ? 1 : 0;
     st->items_processed += (int64_t)ctr[C_ITEMS];
     st->max_frontier = std::max<int64_t>(st->max_frontier, (int64_t)ctr[C_MAX_FRONTIER]);
     st->expand_ms += dp_ms;
@@ -568,6 +749,23 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
     budget = (int64_t)((double)(fr + g->held_bytes()) * 0.8);
   }
   const double frac = kWorklistFraction;
+  // Stored backpointers cost 4 bytes per state; in automatic mode they are
+  // used only when the whole call still fits one admission batch with them.
+  if (g->bp_mode == 0) {
+    g->bp_call = false;
+  } else if (g->bp_mode == 1) {
+    g->bp_call = true;
+  } else {
+    int64_t tot = 0;
+    bool mixed = false;
+    for (const QueryIn& q : qs) {
+      tot += gst_query_footprint(g, q.k, q.D) + bp_bytes(g, q.k, q.D);
+      mixed |= (q.D >= 0) != (qs[0].D >= 0);
+    }
+    const size_t cap_slots =
+        g->max_batch > 0 ? std::min<size_t>(g->max_batch, MAX_SLOTS) : MAX_SLOTS;
+    g->bp_call = !mixed && qs.size() <= cap_slots && tot <= budget;
+  }
   // stable order: plain queries first, then hop queries (one mode per batch)
   std::stable_sort(qs.begin(), qs.end(),
                    [](const QueryIn& a, const QueryIn& b) { return (a.D >= 0) < (b.D >= 0); });
@@ -578,7 +776,8 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
     const bool hop = qs[i].D >= 0;
     const size_t cap_slots = g->max_batch > 0 ? std::min<size_t>(g->max_batch, MAX_SLOTS) : MAX_SLOTS;
     while (i < qs.size() && (qs[i].D >= 0) == hop && batch.size() < cap_slots) {
-      const int64_t f = gst_query_footprint(g, qs[i].k, qs[i].D);
+      const int64_t f = gst_query_footprint(g, qs[i].k, qs[i].D) +
+                        (g->bp_call ? bp_bytes(g, qs[i].k, qs[i].D) : 0);
       if (!batch.empty() && used + f > budget) break;
       if (batch.empty() && f > budget)
         return fail(GST_ERR_OOM, "a single query exceeds the HBM budget");
@@ -593,20 +792,28 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
       // sub-batches that fit the budget at that size
       if (st) ++st->reruns;
       g->release_batch_buffers();  // the held tables and the larger lists never coexist
+      // Sub-batches are bounded by the budget and by the 32-bit list
+      // capacity: their lists hold every state (and chunk) of every query.
       size_t j = 0;
       rc = GST_OK;
       while (j < batch.size() && rc == GST_OK) {
         std::vector<QueryIn> sub;
-        int64_t u = 0;
+        int64_t u = 0, items = 0;
         while (j < batch.size()) {
           const QueryIn& q = batch[j];
-          const int64_t f = gst_query_footprint(g, q.k, q.D) + 32 * slot_items(g, q.k, q.D);
-          if (!sub.empty() && u + f > budget) break;
+          const int64_t it = slot_items(g, q.k, q.D);
+          const int64_t f = gst_query_footprint(g, q.k, q.D) + 32 * it +
+                            (g->bp_call ? bp_bytes(g, q.k, q.D) : 0);
+          if (!sub.empty() && (u + f > budget || items + it > kListClamp)) break;
           u += f;
+          items += it;
           sub.push_back(q);
           ++j;
         }
         rc = solve_lockstep(g, sub, 1.0, out, st);
+        if (rc == 100)  // only a query with more states than the list can address
+          return fail(GST_ERR_OOM, "query " + std::to_string(sub[0].idx) +
+                                       ": its frontier exceeds the 2^32-item worklist");
       }
     }
     if (rc == 100) return fail(GST_ERR_INTERNAL, "worklist overflow at full capacity");
@@ -650,6 +857,63 @@ int upload_rows(gst_graph* g) {
   g->has_eid = true;
   ++g->gen;
   g->drop_graphs();
+  return GST_OK;
+}
+
+// Weight changes of a few CSR entries (caller's indices, new values): the
+// changed entries must come in symmetric pairs (checked by the same hash as
+// at creation, over the changed entries only); only their rows are re-sorted
+// and re-uploaded. The device arrays keep their addresses, so captured
+// graphs stay valid.
+int update_entries(gst_graph* g, std::vector<std::pair<int64_t, int32_t>>& upd) {
+  std::sort(upd.begin(), upd.end(),
+            [](const auto& x, const auto& y) { return x.first < y.first; });
+  std::vector<std::pair<int64_t, int32_t>> ch;
+  for (size_t i = 0; i < upd.size(); ++i) {
+    if (i + 1 < upd.size() && upd[i + 1].first == upd[i].first) continue;  // last value wins
+    if (g->h_w[upd[i].first] != upd[i].second) ch.push_back(upd[i]);
+  }
+  if (ch.empty()) return GST_OK;
+  uint64_t f_out = 0, f_in = 0;
+  std::vector<int32_t> rows;
+  for (const auto& c : ch) {
+    const int32_t v = (int32_t)(std::upper_bound(g->h_rowp.begin(), g->h_rowp.end(),
+                                                 (int32_t)c.first) - g->h_rowp.begin()) - 1;
+    const uint64_t u = (uint64_t)g->h_col[c.first], ww = (uint64_t)c.second;
+    f_out += mix64((uint64_t)v * 0x9E3779B97F4A7C15ull ^ u * 0xC2B2AE3D27D4EB4Full ^
+                   ww * 0x165667B19E3779F9ull);
+    f_in += mix64(u * 0x9E3779B97F4A7C15ull ^ (uint64_t)v * 0xC2B2AE3D27D4EB4Full ^
+                  ww * 0x165667B19E3779F9ull);
+    rows.push_back(v);
+  }
+  if (f_out != f_in)
+    return fail(GST_ERR_INVALID, "weight update is not symmetric: change both entries of an edge");
+  for (const auto& c : ch) g->h_w[c.first] = c.second;
+  std::sort(rows.begin(), rows.end());
+  rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+  std::vector<int32_t> col, w, id;
+  for (int32_t v : rows) {
+    const int32_t b = g->h_rowp[v], e = g->h_rowp[v + 1];
+    for (int32_t j = b; j < e; ++j) g->perm[j] = j;
+    std::sort(g->perm.begin() + b, g->perm.begin() + e, [&](int32_t x, int32_t y) {
+      if (g->h_w[x] != g->h_w[y]) return g->h_w[x] < g->h_w[y];
+      if (g->h_col[x] != g->h_col[y]) return g->h_col[x] < g->h_col[y];
+      return x < y;
+    });
+    col.resize(e - b);
+    w.resize(e - b);
+    id.resize(e - b);
+    for (int32_t j = b; j < e; ++j) {
+      const int32_t o = g->perm[j];
+      col[j - b] = g->h_col[o];
+      w[j - b] = g->h_w[o];
+      id[j - b] = g->h_ids.empty() ? o : g->h_ids[o];
+    }
+    HIP_TRY(hipMemcpy(g->col.as<int32_t>() + b, col.data(), (e - b) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->wt.as<int32_t>() + b, w.data(), (e - b) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(g->eid.as<int32_t>() + b, id.data(), (e - b) * 4, hipMemcpyHostToDevice));
+  }
+  g->maxw = *std::max_element(g->h_w.begin(), g->h_w.end());
   return GST_OK;
 }
 
@@ -764,8 +1028,19 @@ int gst_graph_set_edge_ids(gst_graph* g, const int32_t* ids, int64_t num_undirec
 
 int gst_graph_set_weights(gst_graph* g, const int32_t* w) {
   if (!g || !w) return fail(GST_ERR_INVALID, "bad argument");
-  for (int64_t j = 0; j < g->E; ++j)
+  int64_t changed = 0;
+  for (int64_t j = 0; j < g->E; ++j) {
     if (w[j] < 1 || w[j] > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
+    changed += w[j] != g->h_w[j];
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  if (changed * 8 < g->E) {  // a few entries: re-sort and re-upload their rows only
+    std::vector<std::pair<int64_t, int32_t>> upd;
+    upd.reserve(changed);
+    for (int64_t j = 0; j < g->E; ++j)
+      if (w[j] != g->h_w[j]) upd.push_back({j, w[j]});
+    return update_entries(g, upd);
+  }
   // weights must stay symmetric: entry (v,u) and (u,v) carry the same value
   uint64_t f_out = 0, f_in = 0;
   for (int32_t v = 0; v < g->V; ++v)
@@ -779,6 +1054,20 @@ int gst_graph_set_weights(gst_graph* g, const int32_t* w) {
   if (f_out != f_in) return fail(GST_ERR_INVALID, "weights are not symmetric");
   g->h_w.assign(w, w + g->E);
   return upload_rows(g);
+}
+
+int gst_graph_update_weights(gst_graph* g, const int64_t* csr_index, const int32_t* w,
+                             int64_t n) {
+  if (!g || n < 0 || (n && (!csr_index || !w))) return fail(GST_ERR_INVALID, "bad argument");
+  std::vector<std::pair<int64_t, int32_t>> upd((size_t)n);
+  for (int64_t i = 0; i < n; ++i) {
+    if (csr_index[i] < 0 || csr_index[i] >= g->E)
+      return fail(GST_ERR_INVALID, "CSR index out of range");
+    if (w[i] < 1 || w[i] > (1 << 20)) return fail(GST_ERR_INVALID, "weights must be in [1, 2^20]");
+    upd[i] = {csr_index[i], w[i]};
+  }
+  HIP_TRY(hipSetDevice(g->device));
+  return update_entries(g, upd);
 }
 
 int gst_groups_set(gst_graph* g, const int64_t* off, const int32_t* mem, int32_t G) {
@@ -803,8 +1092,9 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
   if (!g) return fail(GST_ERR_INVALID, "null graph");
   switch (opt) {
     case GST_OPT_CHUNK:
-      if (v < 64 || v > 65536 || (v & (v - 1))) return fail(GST_ERR_INVALID, "chunk must be a power of two in [64, 65536]");
-      g->chunk_shift = ilog2(v);
+      if (v != 0 && (v < 64 || v > 65536 || (v & (v - 1))))
+        return fail(GST_ERR_INVALID, "chunk must be 0 (automatic) or a power of two in [64, 65536]");
+      g->chunk_opt = v ? ilog2(v) : -1;
       break;
     case GST_OPT_GRID:
       if (v < 0 || v > 65536) return fail(GST_ERR_INVALID, "grid out of range");
@@ -859,6 +1149,13 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
       if (v < 1 || v > 100000) return fail(GST_ERR_INVALID, "radius percentage out of range");
       g->radius_pct = (int32_t)v;
       break;
+    case GST_OPT_TRACE:
+      g->trace = v ? 1 : 0;
+      break;
+    case GST_OPT_BACKPOINTERS:
+      if (v < -1 || v > 1) return fail(GST_ERR_INVALID, "backpointer mode is -1, 0 or 1");
+      g->bp_mode = (int32_t)v;
+      break;
     case GST_OPT_DP_DELTA_DIV:
       if (v < -1 || v > (1 << 20)) return fail(GST_ERR_INVALID, "band divisor out of range");
       g->dp_div = (int32_t)v;
@@ -871,6 +1168,15 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
   return GST_OK;
 }
 
+int gst_trace_get(gst_graph* g, gst_trace_row* rows, int64_t cap, int64_t* n) {
+  if (!g || !n || cap < 0 || (cap && !rows)) return fail(GST_ERR_INVALID, "bad argument");
+  *n = (int64_t)g->trace_rows.size();
+  const int64_t m = std::min<int64_t>(cap, *n);
+  for (int64_t i = 0; i < m; ++i) rows[i] = g->trace_rows[i];
+  return m < *n ? fail(GST_ERR_CAPACITY, "trace longer than the buffer; *n holds its length")
+                : GST_OK;
+}
+
 int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
   if (!g || k < 1 || k > GST_MAX_K) return -1;
   const double wl_items =
@@ -880,16 +1186,13 @@ int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
          (int64_t)g->max_tree_edges * 12 + 1024;
 }
 
-int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
-                    const int32_t* q_D, int32_t nq, int64_t budget, int32_t* out_costs,
-                    int32_t* out_status, int64_t* out_edge_offsets, int32_t* out_edge_uv,
-                    int32_t* out_edge_ids, int64_t edge_cap, gst_stats* st) {
-  auto t0 = std::chrono::steady_clock::now();
-  if (!g || nq < 0 || (nq && (!q_offsets || !q_groups || !out_costs)))
-    return fail(GST_ERR_INVALID, "bad argument");
-  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
-  HIP_TRY(hipSetDevice(g->device));
-  std::vector<QueryIn> qs(nq);
+}  // extern "C"
+
+namespace {
+
+int parse_queries(const gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                  const int32_t* q_D, int32_t nq, std::vector<QueryIn>& qs) {
+  qs.resize(nq);
   for (int32_t q = 0; q < nq; ++q) {
     const int64_t k = q_offsets[q + 1] - q_offsets[q];
     const int32_t D = q_D ? q_D[q] : -1;
@@ -906,6 +1209,148 @@ int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_gro
     }
     qs[q] = QueryIn{q, (int32_t)k, D < 0 ? -1 : D, gr};
   }
+  return GST_OK;
+}
+
+// Seconds of one pass over the sample with the settings in `t` (one call
+// with every query, or one call per query when `one_at_a_time`); the best
+// of `reps` passes. Returns < 0 on error.
+double time_pass(gst_graph* g, const std::vector<QueryIn>& qs, bool one_at_a_time,
+                 const Tune& t, int reps, int* rc) {
+  const int32_t save_chunk = g->chunk_opt, save_grid = g->grid, save_dp = g->dp_div,
+                save_ss = g->sssp_band;
+  g->chunk_opt = t.chunk_shift;
+  g->grid = t.grid;
+  g->dp_div = t.dp_div;
+  g->sssp_band = t.sssp_band;
+  double best = 1e30;
+  BatchOut out;
+  out.cost.assign(qs.size(), 0);
+  out.status.assign(qs.size(), 0);
+  out.uv.resize(qs.size());
+  out.ids.resize(qs.size());
+  *rc = GST_OK;
+  for (int r = 0; r < reps && *rc == GST_OK; ++r) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (one_at_a_time) {
+      for (size_t i = 0; i < qs.size() && *rc == GST_OK; ++i) {
+        std::vector<QueryIn> one{qs[i]};
+        one[0].idx = (int32_t)i;
+        select_settings(g, 1);
+        *rc = solve_all(g, one, 0, out, nullptr);
+      }
+    } else {
+      std::vector<QueryIn> all(qs);
+      for (size_t i = 0; i < all.size(); ++i) all[i].idx = (int32_t)i;
+      select_settings(g, (int64_t)all.size());
+      *rc = solve_all(g, all, 0, out, nullptr);
+    }
+    best = std::min(best, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                              .count());
+  }
+  g->chunk_opt = save_chunk;
+  g->grid = save_grid;
+  g->dp_div = save_dp;
+  g->sssp_band = save_ss;
+  return *rc == GST_OK ? best : -1.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gst_autotune(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                 const int32_t* q_D, int32_t nq, int32_t batch_size, double max_seconds,
+                 gst_tuning* out) {
+  if (!g || nq < 0 || (nq && (!q_offsets || !q_groups)) || (batch_size != 0 && batch_size != 1))
+    return fail(GST_ERR_INVALID, "bad argument");
+  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  HIP_TRY(hipSetDevice(g->device));
+  std::vector<QueryIn> qs;
+  int rc = parse_queries(g, q_offsets, q_groups, q_D, nq, qs);
+  if (rc) return rc;
+  const bool single = batch_size == 1;
+  const int cls = single ? 0 : size_class(nq);
+  // start from what is in force for this class: cached choice or model
+  Tune cur = model_settings(g, cls);
+  if (g->tuned_ok[cls]) cur = g->tuned[cls];
+  if (cur.chunk_shift < 0) cur.chunk_shift = model_settings(g, cls).chunk_shift;
+  if (cur.grid <= 0) cur.grid = g->cus * 8;
+  gst_tuning res;
+  std::memset(&res, 0, sizeof(res));
+  res.size_class = cls;
+  if (nq > 0 && max_seconds > 0) {
+    const auto t_start = std::chrono::steady_clock::now();
+    auto elapsed = [&]() {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    };
+    double best = time_pass(g, qs, single, cur, 2, &rc);
+    if (rc) return rc;
+    res.seconds_before = best;
+    int32_t maxdeg = 1;
+    for (int32_t v = 0; v < g->V; ++v)
+      maxdeg = std::max(maxdeg, g->h_rowp[v + 1] - g->h_rowp[v]);
+    const bool any_plain =
+        std::any_of(qs.begin(), qs.end(), [](const QueryIn& q) { return q.D < 0; });
+    // coordinate descent: one parameter at a time, the others at their best
+    for (int param = 0; param < 4; ++param) {
+      std::vector<int32_t> cand;
+      if (param == 0 && g->chunk_opt < 0) {
+        for (int sh = 6; sh <= 14 && (sh <= 6 || (1 << (sh - 1)) < maxdeg); ++sh) cand.push_back(sh);
+      } else if (param == 1 && g->dp_div < 0 && any_plain) {
+        cand = {0, 64, 256, 1024};
+      } else if (param == 2 && g->sssp_band < 0) {
+        cand = {0, 8, 16, 32, 64};
+      } else if (param == 3 && g->grid <= 0) {
+        cand = {g->cus * 4, g->cus * 8, g->cus * 16};
+      }
+      for (int32_t x : cand) {
+        if (elapsed() > max_seconds) break;
+        Tune t = cur;
+        int32_t* f = param == 0 ? &t.chunk_shift
+                                : (param == 1 ? &t.dp_div : (param == 2 ? &t.sssp_band : &t.grid));
+        const int32_t before = *f;
+        *f = x;
+        if (x == before) continue;
+        const double sec = time_pass(g, qs, single, t, 2, &rc);
+        if (rc) return rc;
+        ++res.trials;
+        if (sec < best) {
+          best = sec;
+          cur = t;
+        }
+      }
+    }
+    res.seconds_after = best;
+    g->tuned[cls] = cur;
+    g->tuned_ok[cls] = true;
+  }
+  select_settings(g, single ? 1 : nq);
+  res.chunk = 1 << g->chunk_shift;
+  res.grid = expand_grid(g);
+  res.dp_div = g->dp_div_call;
+  res.sssp_band = g->sssp_band_call;
+  if (out) *out = res;
+  return GST_OK;
+}
+
+int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                    const int32_t* q_D, int32_t nq, int64_t budget, int32_t* out_costs,
+                    int32_t* out_status, int64_t* out_edge_offsets, int32_t* out_edge_uv,
+                    int32_t* out_edge_ids, int64_t edge_cap, gst_stats* st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!g || nq < 0 || (nq && (!q_offsets || !q_groups || !out_costs)))
+    return fail(GST_ERR_INVALID, "bad argument");
+  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  HIP_TRY(hipSetDevice(g->device));
+  std::vector<QueryIn> qs;
+  {
+    int prc = parse_queries(g, q_offsets, q_groups, q_D, nq, qs);
+    if (prc) return prc;
+  }
+  select_settings(g, nq);
+  g->trace_rows.clear();
+  g->batch_no = 0;
   BatchOut out;
   out.cost.assign(nq, 0);
   out.status.assign(nq, 0);
@@ -936,8 +1381,10 @@ int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_gro
   if (out_edge_offsets) out_edge_offsets[nq] = pos;
   local.seconds_total =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  local.bytes_alg = 24 * local.states_expanded + 16 * local.grow_relax + 12 * local.merge_relax +
-                    16 * local.improvements + 32 * local.lb_terms;
+  local.bytes_alg = 16 * local.states_expanded + 12 * local.grow_relax + 8 * local.merge_relax +
+                    12 * local.improvements + 4 * local.lb_terms;
+  local.bytes_layout = 24 * local.states_expanded + 16 * local.grow_relax +
+                       12 * local.merge_relax + 16 * local.improvements + 32 * local.ub_rows;
   if (st) *st = local;
   if (over && (out_edge_uv || out_edge_ids))
     return fail(GST_ERR_CAPACITY, "edge_cap too small; offsets hold the needed size");
@@ -973,7 +1420,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* 
 void gst_graph_destroy(gst_graph* g) {
   if (!g) return;
   g->drop_graphs();
-  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->GMs, &g->far0, &g->far1,
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->GMs, &g->far0, &g->far1,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
                   &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();

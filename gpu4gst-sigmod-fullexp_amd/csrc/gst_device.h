@@ -89,6 +89,9 @@ enum Ctr {
   C_OVERFLOW,
   C_SSSP_UPD,
   C_SSSP_NOUB,
+  C_UBROW,
+  C_BP_MISS,
+  C_ACTIVE,  // slots that expanded items in the last iteration (trace mode)
   C_NUM
 };
 
@@ -102,7 +105,11 @@ struct Sched {
   int32_t thr, mindef, active, delta;
   int32_t R;
   int32_t prev;  // banded one-label phase: the previous band's end
-  int32_t pad[2];
+  // 1 once a relaxation of a finite value produced a sum >= INF (a path or
+  // tree cost the int32 cost range cannot hold): such a query that ends
+  // without a solution is reported as out of range, not as infeasible
+  int32_t sat;
+  int32_t pad;
 };
 
 struct DevCtx {
@@ -148,7 +155,23 @@ struct DevCtx {
   uint64_t* far[2];
   uint32_t* far_cnt;             // [2]
   int32_t* far_sel;
+  // Stored backpointers (null = trees are recomputed from the cost table):
+  // one word per state, indexed like E, written with a plain store after every
+  // successful table decrease (bp_* below). A race can leave the word of an
+  // older write; the tree recovery checks every word against the final table
+  // and searches for a predecessor when the check fails.
+  int32_t* BP;
+  int32_t trace;  // 1 = count the slots active in each iteration (C_ACTIVE)
 };
+
+// Backpointer words: grow from vertex u (bit 31 clear), merge of subset p1 at
+// depth d1 with its complement in the state's subset at depth d2 (bit 31 set),
+// or a base state (all ones).
+constexpr uint32_t BP_BASE = 0xFFFFFFFFu;
+__host__ __device__ inline uint32_t bp_grow(uint32_t u) { return u; }
+__host__ __device__ inline uint32_t bp_merge(uint32_t p1, uint32_t d1, uint32_t d2) {
+  return 0x80000000u | p1 | (d1 << 7) | (d2 << 11);
+}
 
 struct WL {
   const uint64_t* in;
@@ -175,8 +198,11 @@ void launch_vv(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* go
 void launch_route(const DevCtx& c, int nslots, hipStream_t st);
 void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
                     const int32_t* gmem, uint64_t* wl, uint32_t* cnt, hipStream_t st);
+// Tree recovery for nslots slots: slot sel[i] (or i when sel is null) writes
+// its edges to row i of the outputs (max_edges each).
 void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_uv,
-                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, hipStream_t st);
+                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, const int32_t* sel,
+                      hipStream_t st);
 void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
                      float* gw, int64_t nund, hipStream_t st);
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st);

@@ -514,6 +514,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       const uint64_t it = w.in[idx];
       const uint32_t v = it_v(it), ch = it_ch(it), s = it_s(it);
       const SlotDesc S = c.slots[s];
+      if (c.trace && !c.sched[s].active) c.sched[s].active = 1;  // per-iteration trace only
       const int32_t best = S.D < 0 ? best_of(&c.bestw[s]) : INF;
       const int32_t radius = sssp_radius(c, best);
       n_noub += S.D < 0 && radius >= INF;
@@ -650,9 +651,11 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
           // lowered the label in between, u is queued once more than needed,
           // and cand, a real path length, still gives a valid upper bound.
           uint32_t improved = 0;
+          bool sat = false;
 #pragma unroll
           for (int i = 0; i < 8; ++i) {
             const int32_t cand = cv[i] + wgt;  // cv[i] = INF for idle labels: never improves
+            sat = sat || (cv[i] < INF && cand >= INF);
             if (cv[i] < INF && cand < cu[i]) {
               __hip_atomic_fetch_min(&row[i], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
               ++n_upd;
@@ -660,6 +663,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
               cu[i] = cand;
             }
           }
+          if (sat) c.sched[sl].sat = 1;  // a distance past the int32 cost range
           if (improved) {
             if (meta & 1u) {
               // every finite distance is a real path length: u joined to all groups
@@ -703,9 +707,13 @@ void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st) 
 // iteration number, moves each slot's threshold past an exhausted band and
 // records frontier statistics.
 __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_next, int dp) {
+  __shared__ int n_active;
+  if (threadIdx.x == 0) n_active = 0;
+  __syncthreads();
   const int ns = *c.nslots;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
+    if (c.trace && x.active) atomicAdd(&n_active, 1);
     // near/far modes move bands only when the near list has drained (k_far_*)
     const bool fixed = dp ? c.far_mode != 0 : c.sssp_far != 0;
     if (!fixed && !x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
@@ -717,6 +725,7 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
     const uint32_t produced = *cnt_next;
     if (*cnt_consumed > 0) atomicAdd(&c.ctr[dp ? C_DP_ITERS : C_SSSP_ITERS], 1ull);
     if (dp && produced > c.ctr[C_MAX_FRONTIER]) c.ctr[C_MAX_FRONTIER] = produced;
+    if (c.trace) c.ctr[C_ACTIVE] = (unsigned long long)n_active;  // slots that expanded
     *cnt_consumed = 0;
     *c.iter += 1;
   }
@@ -1195,6 +1204,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
         uint64_t item = 0;
         if (live) {
           atomicMin(&c.T[2 * t_ix(S, p, v)], 0);
+          if (c.BP) c.BP[e_ix(S, p, v)] = (int32_t)BP_BASE;
           if ((int32_t)p == S.full) {
             ub_update(&c.bestw[tk.slot], 0, wit_full(v, 0));
           } else {
@@ -1250,7 +1260,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_cost[BLOCK];
   __shared__ int64_t s_dst[BLOCK];     // T offset of the destination row of vertex 0
   __shared__ int32_t s_rows[BLOCK];    // rows per vertex of the item's slot
-  __shared__ uint32_t s_meta[BLOCK];   // slot << 12 | d << 8 | p
+  __shared__ uint32_t s_meta[BLOCK];   // |full ^ p| << 24 | slot << 12 | d << 8 | p
   __shared__ uint32_t s_v[BLOCK];      // merge side: the item's vertex
   __shared__ int32_t s_mlim[BLOCK];    //   largest useful partner cost
   __shared__ uint32_t s_mm[BLOCK];     //   mask | partner depths << 8
@@ -1270,7 +1280,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
-  unsigned long long nF = 0, nRg = 0, nRm = 0, nU = 0, nLb = 0, nItems = 0;
+  // nLb: one-label terms of the completion bounds of improved states
+  // (|full ^ P| per successful decrease, where the reference evaluates its
+  // lower bound); nUb: one-label rows read for upper bounds
+  unsigned long long nF = 0, nRg = 0, nRm = 0, nU = 0, nLb = 0, nUb = 0, nItems = 0;
 
   for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
     const uint32_t idx = tile * BLOCK + tid;
@@ -1314,8 +1327,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           if (t2 < INF && t < INF) {
             const int32_t mcost = t + t2;
             int32_t* dfull = &c.T[2 * t_ix(S, S.full, v)];
-            if (mcost < *dfull && mcost < atomicMin(dfull, mcost))
+            if (mcost < *dfull && mcost < atomicMin(dfull, mcost)) {
+              if (c.BP) c.BP[e_ix(S, S.full, v)] = (int32_t)bp_merge(p, 0, 0);
               ub_update(&c.bestw[s], mcost, wit_full(v, 0));
+            }
           }
         }
         if (!half) expand = false;
@@ -1352,9 +1367,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         deg = (HOP && (int32_t)d >= Dq) ? 0 : max(0, re - rb);
         s_beg[tid] = rb;
         s_cost[tid] = t;
+        s_v[tid] = v;
         s_dst[tid] = S.t_off + (HOP ? ((int64_t)(d + 1) * S.nsub + p) : (int64_t)p);
         s_rows[tid] = S.nrows;
-        s_meta[tid] = (s << 12) | (d << 8) | p;
+        s_meta[tid] = ((uint32_t)__popc((uint32_t)S.full ^ p) << 24) | (s << 12) | (d << 8) | p;
         if (ch == 0) mask = (uint32_t)S.full ^ p;
       }
     }
@@ -1451,10 +1467,11 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         const int j = jq[q];
         if (j >= 0 && t2q[q] < INF && t2q[q] <= s_mlim[j]) {
           const uint32_t meta = s_meta[j];
-          const uint32_t sl = meta >> 12, dj = (meta >> 8) & 0xFu, pj = meta & 0xFFu;
+          const uint32_t sl = (meta >> 12) & 0xFFFu, dj = (meta >> 8) & 0xFu, pj = meta & 0xFFu;
           const uint32_t mk = s_mm[j] & 0xFFu, p2 = p2q[q], P = pj | p2;
           const uint32_t vj = s_v[j];
           const int32_t mc = s_cost[j] + t2q[q];
+          if (mc >= INF) c.sched[sl].sat = 1;  // a tree cost past the int32 cost range
           const uint32_t nd = HOP ? max(dj, d2q[q]) : 0u;
           const int64_t drow = HOP ? (int64_t)nd * (int64_t)((mk | pj) + 1) + P : (int64_t)P;
           int32_t* dst = &c.T[2 * (s_tb[j] + drow)];
@@ -1463,6 +1480,9 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
             if (HOP && p2 == mk) {  // a full tree
               if (mc < atomicMin(dst, mc)) {
                 ++nU;
+                if (c.BP)
+                  c.BP[c.slots[sl].e_off + (s_tb[j] - c.slots[sl].t_off) + drow] =
+                      (int32_t)bp_merge(pj, dj, d2q[q]);
                 ub_update(&c.bestw[sl], mc, wit_full(vj, nd));
               }
             } else {
@@ -1477,9 +1497,11 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               }
               if (keep && mc < atomicMin(dst, mc)) {
                 ++nU;
+                nLb += (uint32_t)__popc(mk ^ p2);  // the improved state's bound terms
                 const SlotDesc Sj = c.slots[sl];
+                if (c.BP) c.BP[e_ix(Sj, drow, vj)] = (int32_t)bp_merge(pj, dj, d2q[q]);
                 if (!HOP) {
-                  ++nLb;
+                  ++nUb;
                   const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)vj * CROW], mk ^ p2,
                                              c.sched[sl].R);
                   if (sumc < INF && mc + sumc < best) {
@@ -1561,9 +1583,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           int32_t* dst = &c.T[2 * (dbase + uoff)];
           const int2 dl = dq[q];
           ++nRg;
+          if (gc >= INF) c.sched[(s_meta[j] >> 12) & 0xFFFu].sat = 1;  // past the cost range
           if (gc < dl.x) {
             const uint32_t meta = s_meta[j];
-            const uint32_t sl = meta >> 12, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
+            const uint32_t sl = (meta >> 12) & 0xFFFu, pd = meta & 0xFFu, dd = (meta >> 8) & 0xFu;
             int32_t best = INF;
             bool keep = true;
             if (!HOP || c.hop_prune) {  // completion bound before the atomic, as for merges
@@ -1572,9 +1595,11 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
             }
             if (keep && gc < atomicMin(dst, gc)) {
               ++nU;
+              nLb += meta >> 24;  // the improved state's bound terms
               const SlotDesc Sj = c.slots[sl];
+              if (c.BP) c.BP[dbase - Sj.t_off + Sj.e_off + uoff] = (int32_t)bp_grow(s_v[j]);
               if (!HOP) {
-                ++nLb;
+                ++nUb;
                 const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)u * CROW],
                                            (uint32_t)Sj.full ^ pd, c.sched[sl].R);
                 if (sumc < INF && gc + sumc < best)
@@ -1607,6 +1632,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   flush(c.ctr, C_RM, nRm);
   flush(c.ctr, C_U, nU);
   flush(c.ctr, C_LB, nLb);
+  flush(c.ctr, C_UBROW, nUb);
   flush(c.ctr, C_ITEMS, nItems);
 }
 
@@ -1641,18 +1667,19 @@ __device__ inline uint32_t deposit(uint32_t r, uint32_t mask) {  // r-th submask
 
 __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32_t* out_uv,
                                                   int32_t* out_eid, int32_t* out_n,
-                                                  int32_t* out_err) {
+                                                  int32_t* out_err, const int32_t* sel) {
   __shared__ int32_t st_v[STACK], st_p[STACK], st_d[STACK], st_b[STACK];
   __shared__ int32_t s_sp, s_ne, s_err;
   __shared__ int32_t s_pick;
-  const int s = blockIdx.x;
+  const int s = sel ? sel[blockIdx.x] : (int)blockIdx.x;
+  const int o = blockIdx.x;  // output row
   const int lane = threadIdx.x;
   const SlotDesc S = c.slots[s];
   const uint64_t bw = c.bestw[s];
   const int32_t cost = (int32_t)(bw >> 32);
   const uint32_t wit = (uint32_t)bw;
-  int32_t* ouv = out_uv + (int64_t)s * max_edges * 2;
-  int32_t* oid = out_eid + (int64_t)s * max_edges;
+  int32_t* ouv = out_uv + (int64_t)o * max_edges * 2;
+  int32_t* oid = out_eid + (int64_t)o * max_edges;
   const bool hop = S.D >= 0;
   if (lane == 0) {
     s_sp = 0;
@@ -1662,8 +1689,8 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
   __syncthreads();
   if (cost >= INF) {
     if (lane == 0) {
-      out_n[s] = 0;
-      out_err[s] = 0;
+      out_n[o] = 0;
+      out_err[o] = 0;
     }
     return;
   }
@@ -1747,6 +1774,70 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     const uint32_t gs = c.GM[(int64_t)s * c.Vp + v];
     __syncthreads();
     if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
+    if (c.BP) {
+      // the stored backpointer, used when it checks out against the final table
+      const uint32_t bp = (uint32_t)c.BP[e_ix(S, (int64_t)d * S.nsub + p, v)];
+      bool used = false;
+      if (bp != BP_BASE && (bp >> 31)) {
+        const uint32_t p1 = bp & 0x7Fu, d1 = (bp >> 7) & 0xFu, d2 = (bp >> 11) & 0xFu;
+        const uint32_t p2 = p ^ p1;
+        const bool shape = p1 != 0 && (p1 & ~p) == 0 && p2 != 0 &&
+                           (hop ? (max(d1, d2) == d && (int32_t)(d1 + d2) <= S.D)
+                                : (d1 == 0 && d2 == 0));
+        if (shape) {
+          const int32_t a1 = Tat(v, p1, d1), a2 = Tat(v, p2, d2);
+          if (a1 < INF && a2 < INF && a1 + a2 <= b) {
+            used = true;
+            if (lane == 0) {
+              if (s_sp + 2 > STACK) {
+                s_err = 3;
+              } else {
+                st_v[s_sp] = (int32_t)v;
+                st_p[s_sp] = (int32_t)p1;
+                st_d[s_sp] = (int32_t)d1;
+                st_b[s_sp] = a1;
+                st_v[s_sp + 1] = (int32_t)v;
+                st_p[s_sp + 1] = (int32_t)p2;
+                st_d[s_sp + 1] = (int32_t)d2;
+                st_b[s_sp + 1] = a2;
+                s_sp += 2;
+              }
+            }
+          }
+        }
+      } else if (bp != BP_BASE && bp < (uint32_t)c.V && (!hop || d >= 1)) {
+        const uint32_t u = bp, pd = hop ? d - 1 : 0;
+        const int32_t tu = Tat(u, p, pd);
+        int32_t pick = -1;
+        if (tu < INF) {
+          // the CSR entry of edge (v, u): rows are sorted by weight, so scan
+          const int32_t rb = c.rowp[v], re = c.rowp[v + 1];
+          for (int32_t jb = rb; jb < re && pick < 0; jb += 64) {
+            const int32_t j = jb + lane;
+            const bool ok = j < re && (uint32_t)c.col[j] == u && tu + c.wt[j] <= b;
+            const uint64_t m = __ballot(ok);
+            if (m) pick = jb + (int32_t)__ffsll((unsigned long long)m) - 1;
+          }
+        }
+        if (pick >= 0) {
+          used = true;
+          if (lane == 0) {
+            emit(v, u, pick);
+            if (s_sp + 1 > STACK) {
+              s_err = 3;
+            } else {
+              st_v[s_sp] = (int32_t)u;
+              st_p[s_sp] = (int32_t)p;
+              st_d[s_sp] = (int32_t)pd;
+              st_b[s_sp] = tu;
+              s_sp += 1;
+            }
+          }
+        }
+      }
+      if (used) continue;
+      if (lane == 0) atomicAdd(&c.ctr[C_BP_MISS], 1ull);  // stale word: search instead
+    }
     // merge splits: proper non-empty submasks p1 (hop: with layers d1, d2,
     // max(d1, d2) = d, d1 + d2 <= D)
     const uint32_t pc = (uint32_t)__popc(p);
@@ -1846,14 +1937,17 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
   }
   __syncthreads();
   if (lane == 0) {
-    out_n[s] = s_ne;
-    out_err[s] = s_err;
+    out_n[o] = s_ne;
+    out_err[o] = s_err;
   }
 }
 
 void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_uv,
-                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, hipStream_t st) {
-  if (nslots > 0) hipLaunchKernelGGL(k_backtrack, nslots, 64, 0, st, c, max_edges, out_uv, out_eid, out_n, out_err);
+                      int32_t* out_eid, int32_t* out_n, int32_t* out_err, const int32_t* sel,
+                      hipStream_t st) {
+  if (nslots > 0)
+    hipLaunchKernelGGL(k_backtrack, nslots, 64, 0, st, c, max_edges, out_uv, out_eid, out_n,
+                       out_err, sel);
 }
 
 // ---------------------------------------------------------------------------

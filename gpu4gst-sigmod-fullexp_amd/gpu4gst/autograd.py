@@ -18,7 +18,7 @@ from .solver import GSTGraph
 
 class GSTCost(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, weights: torch.Tensor, graph: GSTGraph, queries, D=None):
+    def forward(ctx, weights: torch.Tensor, graph: GSTGraph, queries, D=None, keep=None):
         if graph.eid is None:
             raise ValueError("graph needs undirected edge ids (GSTGraph.from_generated)")
         w = weights.detach().to("cpu", torch.float64).numpy()
@@ -33,6 +33,8 @@ class GSTCost(torch.autograd.Function):
         ctx.save_for_backward(torch.from_numpy(res.offsets).to(dev),
                               torch.from_numpy(res.edge_ids).to(dev))
         ctx.result = res
+        if keep is not None:  # the caller's list receives the trees the backward uses
+            keep.append(res)
         return torch.from_numpy(res.costs.astype(np.float32)).to(dev)
 
     @staticmethod
@@ -46,8 +48,13 @@ class GSTCost(torch.autograd.Function):
                                grad_w.data_ptr(), ctx.nund)
         else:
             raise RuntimeError("GSTCost.backward needs CUDA (HIP) tensors")
-        return grad_w, None, None, None
+        return grad_w, None, None, None, None
 
 
-def gst_cost(weights: torch.Tensor, graph: GSTGraph, queries, D=None) -> torch.Tensor:
-    return GSTCost.apply(weights, graph, queries, D)
+def gst_cost(weights: torch.Tensor, graph: GSTGraph, queries, D=None, keep=None) -> torch.Tensor:
+    """Optimal costs of `queries` as a float32 tensor [nq] differentiable in
+    `weights` (float32 [num_undirected], integer values >= 1). When `keep` is a
+    list, the forward's BatchResult (costs, statuses and the trees whose
+    incidence the backward scatters) is appended to it: among tied optimal
+    trees the solver may pick a different one on another call."""
+    return GSTCost.apply(weights, graph, queries, D, keep)

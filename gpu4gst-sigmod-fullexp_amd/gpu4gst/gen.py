@@ -34,6 +34,8 @@ def _lib():
         L.gg_from_edges.restype = vp
         L.gg_from_edges.argtypes = [i64, i64, vp, vp, vp]
         L.gg_keep_lcc.argtypes = [vp]
+        L.gg_lcc_vertices.restype = i64
+        L.gg_lcc_vertices.argtypes = [vp, vp]
         L.gg_set_weights_uniform.argtypes = [vp, i32, i32, u64]
         L.gg_num_vertices.restype = i64
         L.gg_num_vertices.argtypes = [vp]
@@ -46,6 +48,8 @@ def _lib():
         L.gg_groups_poisson.argtypes = [i64, i32, dbl, u64]
         L.gg_groups_zipf.restype = vp
         L.gg_groups_zipf.argtypes = [i64, i32, dbl, i32, i32, u64]
+        L.gg_groups_zipf_pool.restype = vp
+        L.gg_groups_zipf_pool.argtypes = [i64, vp, i64, i32, dbl, i32, i32, u64]
         L.gg_groups_count.restype = i64
         L.gg_groups_count.argtypes = [vp]
         L.gg_groups_total.restype = i64
@@ -75,17 +79,25 @@ class Graph:
     col: np.ndarray  # int32 [2E]
     cw: np.ndarray  # int32 [2E]
     eid: np.ndarray  # int32 [2E] undirected edge id of each CSR entry
+    # vertices of the largest component (increasing ids); set when the graph
+    # keeps all its vertices (lcc=False), None when it is the component itself
+    lcc: np.ndarray | None = None
 
     @property
     def m(self) -> int:
         return int(self.src.shape[0])
 
 
-def _finish(h, lo: int, hi: int, wseed: int, lcc: bool) -> Graph:
+def _finish(h, lo: int, hi: int, wseed: int, lcc: bool, pool: bool = False) -> Graph:
     L = _lib()
+    comp = None
     try:
         if lcc:
             L.gg_keep_lcc(h)
+        elif pool:
+            nc = L.gg_lcc_vertices(h, None)
+            comp = np.empty(nc, np.int32)
+            L.gg_lcc_vertices(h, _p(comp))
         if lo is not None:
             L.gg_set_weights_uniform(h, lo, hi, wseed)
         n = L.gg_num_vertices(h)
@@ -101,23 +113,26 @@ def _finish(h, lo: int, hi: int, wseed: int, lcc: bool) -> Graph:
         L.gg_get_csr(h, _p(beg), _p(col), _p(cw), _p(eid))
     finally:
         L.gg_free(h)
-    return Graph(int(n), src, dst, w, beg, col, cw, eid)
+    return Graph(int(n), src, dst, w, beg, col, cw, eid, comp)
 
 
 def rmat(scale: int, m: int, a=0.57, b=0.19, c=0.19, *, seed: int, wlo=1, whi=100,
          wseed: int | None = None, lcc=True) -> Graph:
+    """lcc=True keeps the largest component (relabelled 0..V'-1); lcc=False
+    keeps all 2^scale vertices and records the component in Graph.lcc."""
     h = _lib().gg_rmat(scale, m, a, b, c, seed)
     if not h:
         raise ValueError("invalid R-MAT parameters")
-    return _finish(h, wlo, whi, seed + 1 if wseed is None else wseed, lcc)
+    return _finish(h, wlo, whi, seed + 1 if wseed is None else wseed, lcc, pool=not lcc)
 
 
 def chung_lu(n: int, m: int, beta: float, dmax: float, *, seed: int, wlo=1, whi=1000,
              wseed: int | None = None, lcc=True) -> Graph:
+    """lcc as for rmat()."""
     h = _lib().gg_chung_lu(n, m, beta, dmax, seed)
     if not h:
         raise ValueError("invalid Chung-Lu parameters")
-    return _finish(h, wlo, whi, seed + 1 if wseed is None else wseed, lcc)
+    return _finish(h, wlo, whi, seed + 1 if wseed is None else wseed, lcc, pool=not lcc)
 
 
 def from_edges(n: int, src, dst, w=None, lcc=False) -> Graph:
@@ -168,6 +183,16 @@ def groups_poisson(n: int, G: int, lam: float, *, seed: int) -> Groups:
 
 def groups_zipf(n: int, G: int, s: float, smin: int, smax: int, *, seed: int) -> Groups:
     h = _lib().gg_groups_zipf(n, G, s, smin, smax, seed)
+    if not h:
+        raise ValueError("invalid group parameters")
+    return _groups(h)
+
+
+def groups_zipf_pool(n: int, pool, G: int, s: float, smin: int, smax: int, *,
+                     seed: int) -> Groups:
+    """groups_zipf with members drawn from the vertex ids in `pool` only."""
+    pool = np.ascontiguousarray(pool, np.int32)
+    h = _lib().gg_groups_zipf_pool(n, _p(pool), pool.shape[0], G, s, smin, smax, seed)
     if not h:
         raise ValueError("invalid group parameters")
     return _groups(h)

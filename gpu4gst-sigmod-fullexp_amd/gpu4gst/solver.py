@@ -17,7 +17,10 @@ import numpy as np
 
 from . import _lib
 
-STATUS_OK, STATUS_INFEASIBLE = 0, 1
+# per-query status (include/gst.h GST_Q_*): optimal tree found; no tree covers
+# the groups; the optimum (if any) lies beyond the int32 cost range; cost exact
+# but the tree too long for the recovery buffer
+STATUS_OK, STATUS_INFEASIBLE, STATUS_RANGE, STATUS_NO_TREE = 0, 1, 2, 3
 
 
 class GSTError(RuntimeError):
@@ -107,6 +110,12 @@ class GSTGraph:
             raise ValueError("weights must have one entry per CSR entry")
         _check(_lib.lib().gst_graph_set_weights(self._h, _p(w)))
 
+    def update_weights(self, csr_index, w):
+        """weight[csr_index[i]] = w[i]; both entries of an edge must change together."""
+        idx = np.ascontiguousarray(csr_index, np.int64)
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(w, np.int32), idx.shape), np.int32)
+        _check(_lib.lib().gst_graph_update_weights(self._h, _p(idx), _p(w), idx.shape[0]))
+
     def set_option(self, name: str, value: int):
         _check(_lib.lib().gst_set_option(self._h, _lib.OPT[name], int(value)))
 
@@ -154,6 +163,37 @@ class GSTGraph:
         tot = int(eoff[-1])
         return BatchResult(costs, status, eoff, uv[:2 * tot].reshape(-1, 2).copy(),
                            ids[:tot].copy(), st.as_dict())
+
+    def autotune(self, queries, D=None, batch_size: int = 0, max_seconds: float = 10.0) -> dict:
+        """Times candidate kernel settings on `queries` and keeps the fastest for
+        their size class on this graph (gst_autotune); batch_size = 1 tunes the
+        one-query path. max_seconds <= 0 only reports the settings in force."""
+        qs = [np.ascontiguousarray(q, np.int32) for q in queries]
+        nq = len(qs)
+        off = np.zeros(nq + 1, np.int64)
+        for i, q in enumerate(qs):
+            off[i + 1] = off[i] + q.shape[0]
+        groups = np.concatenate(qs) if nq else np.zeros(0, np.int32)
+        Dv = None
+        if D is not None:
+            Dv = np.ascontiguousarray(np.broadcast_to(np.asarray(D, np.int32), (nq,)), np.int32)
+        out = _lib.GstTuning()
+        _check(_lib.lib().gst_autotune(self._h, _p(off), _p(groups), _p(Dv), nq, batch_size,
+                                       float(max_seconds), C.byref(out)))
+        return out.as_dict()
+
+    def trace(self) -> list[dict]:
+        """Per-launch trace of the last solve_batch run with options profile=1 and
+        trace=1: phase, batch, items, active slots, slots, ms."""
+        L = _lib.lib()
+        n = C.c_int64(0)
+        rc = L.gst_trace_get(self._h, None, 0, C.byref(n))  # the length only
+        if rc != _lib.ERR_CAPACITY:
+            _check(rc)
+        rows = (_lib.GstTraceRow * max(1, n.value))()
+        _check(L.gst_trace_get(self._h, rows, n.value, C.byref(n)))
+        return [{f: getattr(r, f) for f, _ in _lib.GstTraceRow._fields_ if f != "pad"}
+                for r in rows[:n.value]]
 
     def backward(self, nq: int, d_offsets: int, d_ids: int, d_grad_out: int, d_grad_w: int,
                  num_undirected: int):

@@ -13,8 +13,15 @@
  4  Chung-Lu V = 317,000, 1.05M edges, exponent 2.4 (largest expected degree
     3,000); weights U[1,1000]; 5,000 groups Zipf(1.2) in [1,5000]; 256 queries
     of k = 5, differentiable.
- 5  R-MAT scale 22, 35M distinct edges; weights U[1,1000]; 50,000 groups
-    Zipf(1.2) in [1,20000]; 256 queries of k = 7.
+ 5  R-MAT scale 22 (V = 4,194,304), 35M distinct edges; weights U[1,1000];
+    50,000 groups Zipf(1.2) in [1,20000]; 256 queries of k = 7.
+
+Workloads 1 and 2 keep only the largest component, as their definitions say.
+Workloads 3, 4 and 5 keep every vertex (their stated V, so every per-query
+table has its full size: 4,194,304 x 128 states per query on workload 5),
+and draw group members uniformly from the vertices of the largest component,
+so that every query is feasible (on R-MAT scale 22 half of the vertices lie
+outside it, most of them isolated).
 
 Streams of seeds: seed(cfg, stream) = 0x67535400 + 16 cfg + stream, stream 0
 graph, 1 weights, 2 groups, 3 queries, 4 hop bounds / sampling.
@@ -61,21 +68,21 @@ def config(cfg: int, n_queries: int | None = None) -> Workload:
         return Workload("chunglu168k_k357", g, grp, _queries(grp, ks, 2), None)
     if cfg == 3:
         g = gen.chung_lu(1100000, 3000000, 2.3, 5000.0, seed=seed(3, 0), wlo=1, whi=100,
-                         wseed=seed(3, 1))
-        grp = gen.groups_zipf(g.n, 20000, 1.3, 1, 2000, seed=seed(3, 2))
+                         wseed=seed(3, 1), lcc=False)
+        grp = gen.groups_zipf_pool(g.n, g.lcc, 20000, 1.3, 1, 2000, seed=seed(3, 2))
         nq = n_queries or 512
         D = gen.uniform_ints(nq, 3, 5, seed=seed(3, 4))
         return Workload("chunglu1100k_k5_hop", g, grp, _queries(grp, [5] * nq, 3), D)
     if cfg == 4:
         g = gen.chung_lu(317000, 1050000, 2.4, 3000.0, seed=seed(4, 0), wlo=1, whi=1000,
-                         wseed=seed(4, 1))
-        grp = gen.groups_zipf(g.n, 5000, 1.2, 1, 5000, seed=seed(4, 2))
+                         wseed=seed(4, 1), lcc=False)
+        grp = gen.groups_zipf_pool(g.n, g.lcc, 5000, 1.2, 1, 5000, seed=seed(4, 2))
         nq = n_queries or 256
         return Workload("chunglu317k_k5_grad", g, grp, _queries(grp, [5] * nq, 4), None)
     if cfg == 5:
         g = gen.rmat(22, 35000000, 0.57, 0.19, 0.19, seed=seed(5, 0), wlo=1, whi=1000,
-                     wseed=seed(5, 1))
-        grp = gen.groups_zipf(g.n, 50000, 1.2, 1, 20000, seed=seed(5, 2))
+                     wseed=seed(5, 1), lcc=False)
+        grp = gen.groups_zipf_pool(g.n, g.lcc, 50000, 1.2, 1, 20000, seed=seed(5, 2))
         nq = n_queries or 256
         return Workload("rmat22_k7", g, grp, _queries(grp, [7] * nq, 5), None)
     raise ValueError(f"unknown workload {cfg}")

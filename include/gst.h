@@ -54,6 +54,12 @@ extern "C" {
 /* per-query status in gst_solve_batch's out_status */
 #define GST_Q_OK 0
 #define GST_Q_INFEASIBLE 1   /* no tree covers all query groups */
+#define GST_Q_RANGE 2        /* no tree of cost below 2^29 was found, and some path or tree
+                                cost reached 2^29 while searching: the optimum, if any, lies
+                                beyond the int32 cost range; cost = the infeasible cost */
+#define GST_Q_NO_TREE 3      /* the cost is exact, but the tree has more edges than the
+                                recovery buffer holds (after one retry with room for
+                                min(cost, 4V) edges); no edges are returned */
 
 #define GST_MAX_K 7           /* up to 128 group subsets per query */
 #define GST_MAX_D 15          /* hop bound */
@@ -73,10 +79,12 @@ typedef struct gst_stats {
   int64_t grow_relax;         /* Rg: DP edge relaxations */
   int64_t merge_relax;        /* Rm: DP merge candidates evaluated */
   int64_t improvements;       /* U: successful DP table decreases */
-  int64_t lb_terms;           /* Lb: one-label rows read for upper bounds (32 B each) */
+  int64_t lb_terms;           /* Lb: one-label terms of the lower bounds of improved states
+                                 (|full \ P| per successful DP table decrease) */
   int64_t items_processed;    /* DP worklist items (chunks included) */
   int64_t max_frontier;       /* largest DP worklist */
-  int64_t bytes_alg;          /* algorithmic bytes of the DP expand kernel */
+  int64_t bytes_alg;          /* algorithmic bytes of the DP expand kernel:
+                                 16 F + 12 Rg + 8 Rm + 12 U + 4 Lb */
   double seconds_total;       /* host wall clock of the call */
   double expand_ms;           /* DP expand kernel time (profiling mode only) */
   int64_t expand_launches;    /* DP expand launches (profiling mode only) */
@@ -84,12 +92,19 @@ typedef struct gst_stats {
   double phase_ms[3];         /* device time: one-label phase, bounds + seeds, DP phase */
   int64_t sssp_expand_launches; /* one-label expand launches (profiling mode only) */
   int64_t reruns;             /* batches re-run with full-size lists after a list overflowed */
+  int64_t ub_rows;            /* one-label rows read for DP upper bounds (32 B each) */
+  int64_t bytes_layout;       /* bytes the DP expand kernel's own layout moves for the same
+                                 work: 24 F + 16 Rg + 12 Rm + 16 U + 32 ub_rows (8-byte
+                                 state words, 64-bit items) */
+  int64_t backpointers_stored; /* 1 if the last batch stored backpointers */
+  int64_t bp_fallbacks;       /* tree-recovery steps whose stored backpointer did not match
+                                 the final table (a racing write) and were searched instead */
 } gst_stats;
 
 /* options for gst_set_option */
 #define GST_OPT_CHUNK 1            /* edges per virtual-split chunk (power of two, 64..65536;
-                                      default 4096) */
-#define GST_OPT_GRID 2             /* workgroups of the expand kernels (0 = auto) */
+                                      0 = automatic, the default: see gst_autotune) */
+#define GST_OPT_GRID 2             /* workgroups of the expand kernels (0 = the autotuner's) */
 #define GST_OPT_ITERS_PER_SYNC 3   /* max graph-captured iterations between host checks */
 #define GST_OPT_HOP_PRUNE 4        /* 1 = lower-bound pruning in the hop DP (default 1) */
 #define GST_OPT_PROFILE 5          /* 1 = time every expand launch with HIP events */
@@ -111,6 +126,14 @@ typedef struct gst_stats {
                                       the usual floors (0 = default: 4%, at least one item per
                                       vertex and query); a list that overflows makes the batch
                                       re-run with full-size lists (gst_stats.reruns) */
+#define GST_OPT_BACKPOINTERS 19    /* tree recovery: 1 = from backpointers stored during the DP
+                                      (one int32 per state, written after every table decrease,
+                                      checked against the final table when read), 0 = recomputed
+                                      from the cost table alone, -1 (default) = stored when the
+                                      whole call fits one admission batch with them, else
+                                      recomputed (memory is tight) */
+#define GST_OPT_TRACE 20           /* 1 = with GST_OPT_PROFILE, record one gst_trace_row per
+                                      expand launch (read with gst_trace_get) */
 /* Ablation switches (the reference's TrimCDP-WB / D-TrimCDP-WB ablation variants, README.md:
  * 181-302); results are the same, only the speed changes: */
 #define GST_OPT_LANE_APPENDS 17    /* 1 = every lane appends its frontier items with its own
@@ -130,8 +153,16 @@ int gst_graph_create(const void* beg_pos, const void* csr, const void* weight, i
 /* Maps each CSR entry to an undirected edge id in [0, num_undirected); used
  * for tree edge ids and by gst_backward. Without it the id is the CSR index. */
 int gst_graph_set_edge_ids(gst_graph* g, const int32_t* csr_edge_id, int64_t num_undirected);
-/* Replaces the weights (CSR order, int32, >= 1). */
+/* Replaces the weights (CSR order, int32, >= 1). When few entries change,
+ * only their rows are re-sorted and re-uploaded. */
 int gst_graph_set_weights(gst_graph* g, const int32_t* weight);
+/* Sets weight[csr_index[i]] = w[i] for i < n (the caller's CSR indices; a
+ * repeated index takes its last value). Both entries of an undirected edge
+ * must change together (GST_ERR_INVALID otherwise, nothing changed). For
+ * re-solves under small perturbations (finite differences of the
+ * differentiable cost); no counterpart in the reference, whose graphs are
+ * read once per run. */
+int gst_graph_update_weights(gst_graph* g, const int64_t* csr_index, const int32_t* w, int64_t n);
 /* Vertex groups: members of group i are group_members[group_offsets[i] ..
  * group_offsets[i+1]); group ids are 0-based (".g" line "gN" is id N-1). */
 int gst_groups_set(gst_graph* g, const int64_t* group_offsets, const int32_t* group_members,
@@ -172,7 +203,55 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_edge_offsets,
                  const int32_t* d_edge_ids, const float* d_grad_out, float* d_grad_w,
                  int64_t num_undirected);
 
-/* Number of device bytes one query of k groups and hop bound D occupies. */
+/* Autotuner. The expand kernels' settings (virtual-split chunk, workgroup
+ * count, first DP band divisor, first one-label band width) are chosen per
+ * size class of a call: one query, 2..31 queries, 32 or more. By default a
+ * model of the class picks them (chunk 256 / 1024 / 4096 edges, 8
+ * workgroups per CU, the band rules of GST_OPT_DP_DELTA_DIV and
+ * GST_OPT_SSSP_DELTA); options set with gst_set_option always win.
+ * gst_autotune measures instead: on the given sample of queries it times
+ * candidate values of every setting the caller has not fixed, one setting at
+ * a time (coordinate descent, best of two passes per candidate, stopping
+ * after about max_seconds), and keeps the fastest for that size class on
+ * this graph; later calls of the class use it. batch_size = 1 tunes the
+ * one-query class (the sample is solved one query per call); 0 tunes the
+ * class of nq (the sample is solved in one call). max_seconds <= 0 only
+ * reports the settings in force for the class. Replaces the reference's
+ * compile-time constants: the degree bins SML_MID / MID_LRG and thread/block
+ * counts (code/TrimCDP-WB/include/util.h:19-20, 31-33) and the virtual-split
+ * threshold and slice (code/TrimCDP-WB/include/graph.hpp:141, 155). */
+typedef struct gst_tuning {
+  int32_t chunk;            /* edges per virtual-split chunk in force */
+  int32_t grid;             /* workgroups of the expand kernels in force */
+  int32_t dp_div;           /* first DP band divisor (-1 = the default rule, 0 = unordered) */
+  int32_t sssp_band;        /* first one-label band (-1 = the default rule, 0 = unordered) */
+  int32_t size_class;       /* 0: one query, 1: 2..31, 2: 32 or more */
+  int32_t trials;           /* candidate settings timed */
+  double seconds_before;    /* one pass over the sample with the settings before tuning */
+  double seconds_after;     /* ... with the chosen settings */
+} gst_tuning;
+int gst_autotune(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups,
+                 const int32_t* q_D, int32_t nq, int32_t batch_size, double max_seconds,
+                 gst_tuning* out);
+
+/* Per-iteration trace of the last gst_solve_batch call made with
+ * GST_OPT_PROFILE and GST_OPT_TRACE set: one row per expand launch. Copies
+ * min(cap, length) rows and sets *n to the length (GST_ERR_CAPACITY when cap
+ * is smaller). No reference counterpart (its harness records one time per
+ * query, code/TrimCDP/include/exp_GPU_nonHop.h:128). */
+typedef struct gst_trace_row {
+  int32_t phase;          /* 0 one-label, 1 plain DP, 2 hop DP */
+  int32_t batch;          /* admission batch of the call, from 1 */
+  int64_t items;          /* worklist items the launch consumed */
+  int32_t active_slots;   /* queries of the batch that expanded at least one item */
+  int32_t slots;          /* queries in the batch */
+  float ms;               /* the launch's device time */
+  int32_t pad;
+} gst_trace_row;
+int gst_trace_get(gst_graph* g, gst_trace_row* rows, int64_t cap, int64_t* n);
+
+/* Number of device bytes one query of k groups and hop bound D occupies
+ * (with the settings in force for the last call). */
 int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D);
 /* Thread-local description of the last error. */
 const char* gst_last_error(void);

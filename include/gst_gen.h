@@ -32,6 +32,9 @@ gg_graph* gg_from_edges(int64_t n, int64_t m, const int32_t* src, const int32_t*
 /* Keep the largest connected component, relabelling vertices 0..V'-1 in
  * increasing order of their old ids. */
 void gg_keep_lcc(gg_graph* g);
+/* Vertices of the largest connected component (ids unchanged, increasing)
+ * written to out (may be NULL); returns their count. */
+int64_t gg_lcc_vertices(const gg_graph* g, int32_t* out);
 /* Weight every undirected edge uniformly in [lo, hi] (edge-list order). */
 void gg_set_weights_uniform(gg_graph* g, int32_t lo, int32_t hi, uint64_t seed);
 int64_t gg_num_vertices(const gg_graph* g);
@@ -47,6 +50,10 @@ gg_groups* gg_groups_poisson(int64_t n, int32_t G, double lambda, uint64_t seed)
 /* G groups with Zipf(s) sizes clipped to [smin, smax]; members are distinct
  * uniform vertices, stored sorted. */
 gg_groups* gg_groups_zipf(int64_t n, int32_t G, double s, int32_t smin, int32_t smax, uint64_t seed);
+/* The same with members drawn from pool[0 .. npool) (distinct vertex ids in
+ * [0, n)) instead of from all n vertices; pool = NULL means all vertices. */
+gg_groups* gg_groups_zipf_pool(int64_t n, const int32_t* pool, int64_t npool, int32_t G, double s,
+                               int32_t smin, int32_t smax, uint64_t seed);
 int64_t gg_groups_count(const gg_groups* g);
 int64_t gg_groups_total(const gg_groups* g);
 void gg_groups_get(const gg_groups* g, int64_t* off, int32_t* mem);

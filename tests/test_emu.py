@@ -35,6 +35,20 @@ def test_emulated_kernels_match_oracle():
     assert out["fails"] == [], out["fails"]
 
 
+def test_emulated_error_and_range_handling():
+    """Long trees are recovered by the retry, costs past the int32 range are
+    reported as such, sparse weight updates are checked and exact
+    (tests/emu_errors.py)."""
+    _need_emu()
+    env = dict(os.environ, GPU4GST_LIB=EMU)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu_errors.py")],
+                       env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["cases"] == 65
+    assert out["fails"] == [], out["fails"]
+
+
 def _adapter_demo(tmp_path):
     exe = tmp_path / "adapter_demo"
     cc = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
