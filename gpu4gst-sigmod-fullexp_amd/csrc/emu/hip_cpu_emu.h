@@ -46,6 +46,7 @@ inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 #define __shared__ static
 #define __launch_bounds__(...)
 #define __forceinline__ inline
+#define __align__(n) alignas(n)
 #define __HIP_MEMORY_SCOPE_AGENT 0
 
 typedef int hipError_t;
@@ -394,6 +395,11 @@ inline hipError_t hipMemcpy2D(void* d, size_t dpitch, const void* s, size_t spit
   for (size_t r = 0; r < height; ++r)
     std::memcpy(static_cast<char*>(d) + r * dpitch, static_cast<const char*>(s) + r * spitch,
                 width);
+  return hipSuccess;
+}
+#define HIP_SYMBOL(x) (&(x))
+inline hipError_t hipGetSymbolAddress(void** p, const void* sym) {
+  *p = const_cast<void*>(sym);
   return hipSuccess;
 }
 inline hipError_t hipMemset(void* d, int v, size_t n) {

@@ -36,8 +36,10 @@ int fail(int code, const std::string& msg) {
 struct DBuf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;  // a view into the allocation-free path's scratch: never (re)allocated
   hipError_t ensure(size_t want, bool* grew = nullptr) {
     if (want <= bytes && p) return hipSuccess;
+    if (borrowed) return hipErrorOutOfMemory;
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
@@ -48,9 +50,10 @@ struct DBuf {
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p && !borrowed) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
+    borrowed = false;
   }
   template <class T>
   T* as() const {
@@ -125,6 +128,9 @@ struct gst_graph {
   bool bp_call = false;  // in force for the call being solved
   // per-iteration trace of the profiled path (GST_OPT_TRACE with GST_OPT_PROFILE)
   int32_t trace = 0;
+  // inside gst_solve_noalloc: no allocation, no graph capture, no graph
+  // destruction (stale captured graphs are replaced lazily by later calls)
+  bool noalloc = false;
   int32_t batch_no = 0;
   std::vector<gst_trace_row> trace_rows;
   int32_t iters_per_sync = 32;
@@ -298,7 +304,7 @@ void select_settings(gst_graph* g, int64_t nq) {
     g->chunk_shift = cs;
     g->grid_eff = gr;
     ++g->gen;
-    g->drop_graphs();
+    if (!g->noalloc) g->drop_graphs();
   }
 }
 
@@ -521,7 +527,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(g->out_err.ensure((size_t)nslots * 4, &grew));
   if (grew) {
     ++g->gen;
-    g->drop_graphs();
+    if (!g->noalloc) g->drop_graphs();
   }
   HIP_TRY(hipMemcpyAsync(g->slots.p, sd.data(), nslots * sizeof(SlotDesc), hipMemcpyHostToDevice,
                          g->stream));
@@ -619,7 +625,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   // room for min(cost, 4V) edges: every recovered edge weighs >= 1 and their
   // total is at most the cost.
   std::vector<int32_t> redo;
-  for (int s = 0; s < nslots; ++s)
+  for (int s = 0; s < nslots && !g->noalloc; ++s)
     if (oerr[s] == 2) redo.push_back(s);
   std::map<int, std::pair<std::vector<int32_t>, std::vector<int32_t>>> long_trees;
   if (!redo.empty()) {
@@ -1403,6 +1409,153 @@ int gst_solve(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D, i
   if (out_num_edges) *out_num_edges = (int32_t)eoff[1];
   if (out_status) *out_status = status;
   return rc;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Carves the allocation-free path's buffers from the scratch array, 256-byte
+// aligned, in a fixed order; `base` null only measures.
+struct Carve {
+  char* base;
+  size_t used = 0;
+  void take(DBuf& b, size_t bytes) {
+    used = (used + 255) & ~(size_t)255;
+    b.p = base ? base + used : nullptr;
+    b.bytes = bytes;
+    b.borrowed = true;
+    used += bytes;
+  }
+};
+
+// The buffers of one slot at full worklist capacity (lists never overflow).
+void carve_slot(gst_graph* g, int32_t k, int32_t D, DBuf* bufs[], Carve& cv) {
+  const int64_t rows = rows_of(k, D);
+  const int64_t items = slot_items(g, k, D);
+  const int64_t cap = std::max<int64_t>(items, g->nch()) + 4096;
+  const size_t sizes[] = {(size_t)rows * g->Vp * 8,        // T
+                          (size_t)rows * g->Vp * 4,        // Es
+                          (size_t)g->Vp * CROW * 4,        // C
+                          (size_t)g->Vp * 4,               // GMs
+                          (size_t)RTABLE * 4,              // R
+                          (size_t)VV * 4,                  // VVb
+                          8,                               // bestw
+                          sizeof(SlotDesc),                // slots
+                          sizeof(Sched),                   // sched
+                          (size_t)k * sizeof(Task) + 16,   // tasks
+                          (size_t)cap * 8, (size_t)cap * 8, (size_t)cap * 8, (size_t)cap * 8,
+                          (size_t)g->max_tree_edges * 8,   // out_uv
+                          (size_t)g->max_tree_edges * 4,   // out_eid
+                          4, 4};                           // out_n, out_err
+  for (int i = 0; i < 18; ++i) cv.take(*bufs[i], sizes[i]);
+}
+
+DBuf* const* slot_bufs(gst_graph* g, DBuf* out[18]) {
+  DBuf* b[18] = {&g->T,    &g->Es,   &g->C,     &g->GMs,  &g->R,    &g->VVb,
+                 &g->bestw, &g->slots, &g->sched, &g->tasks, &g->wl0,  &g->wl1,
+                 &g->far0,  &g->far1,  &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
+  for (int i = 0; i < 18; ++i) out[i] = b[i];
+  return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t gst_noalloc_bytes(gst_graph* g, int32_t k, int32_t D) {
+  if (!g || k < 1 || k > GST_MAX_K || D > GST_MAX_D) return -1;
+  const bool save = g->noalloc;
+  g->noalloc = true;
+  select_settings(g, 1);
+  g->noalloc = save;
+  DBuf tmp[18];
+  DBuf* bufs[18];
+  for (int i = 0; i < 18; ++i) bufs[i] = &tmp[i];
+  Carve cv{nullptr};
+  carve_slot(g, k, D < 0 ? -1 : D, bufs, cv);
+  return (int64_t)cv.used;
+}
+
+int gst_solve_noalloc(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D,
+                      int32_t* out_cost, int32_t* out_edge_uv, int32_t edge_cap,
+                      int32_t* out_num_edges, int32_t* out_status, gst_stats* st) {
+  auto t0 = std::chrono::steady_clock::now();
+  if (!g || !query_groups || !out_cost || edge_cap < 0) return fail(GST_ERR_INVALID, "bad argument");
+  if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  HIP_TRY(hipSetDevice(g->device));
+  const int64_t off[2] = {0, k};
+  std::vector<QueryIn> qs;
+  int rc = parse_queries(g, off, query_groups, &D, 1, qs);
+  if (rc) return rc;
+  void* base = nullptr;
+  HIP_TRY(noalloc_scratch(&base));
+  g->noalloc = true;
+  select_settings(g, 1);
+  DBuf* bufs[18];
+  slot_bufs(g, bufs);
+  Carve measure{nullptr};
+  {
+    DBuf tmp[18];
+    DBuf* tb[18];
+    for (int i = 0; i < 18; ++i) tb[i] = &tmp[i];
+    carve_slot(g, qs[0].k, qs[0].D, tb, measure);
+  }
+  if (measure.used > (size_t)GST_NOALLOC_SCRATCH_BYTES) {
+    g->noalloc = false;
+    return fail(GST_ERR_OOM, "query needs " + std::to_string(measure.used) +
+                                 " bytes of scratch; the allocation-free path holds " +
+                                 std::to_string((unsigned long long)GST_NOALLOC_SCRATCH_BYTES));
+  }
+  // the graph's own buffers step aside for the call and come back unchanged
+  DBuf saved[18];
+  for (int i = 0; i < 18; ++i) {
+    saved[i] = *bufs[i];
+    *bufs[i] = DBuf();
+  }
+  const int32_t save_graphs = g->use_graphs;
+  const bool save_bp = g->bp_call;
+  const uint32_t save_cap = g->wl_cap;
+  g->use_graphs = 0;
+  g->bp_call = false;
+  Carve cv{static_cast<char*>(base)};
+  carve_slot(g, qs[0].k, qs[0].D, bufs, cv);
+  // the array keeps the last call's contents: clear the enqueue stamps (the
+  // other buffers are written whole before they are read)
+  launch_fill_i32(g->Es.as<int32_t>(), (int64_t)(g->Es.bytes / 4), 0, g->stream);
+  BatchOut out;
+  out.cost.assign(1, 0);
+  out.status.assign(1, 0);
+  out.uv.resize(1);
+  out.ids.resize(1);
+  gst_stats local;
+  std::memset(&local, 0, sizeof(local));
+  rc = solve_lockstep(g, qs, 1.0, out, &local);
+  for (int i = 0; i < 18; ++i) *bufs[i] = saved[i];
+  g->use_graphs = save_graphs;
+  g->bp_call = save_bp;
+  g->wl_cap = save_cap;
+  ++g->gen;  // the next regular call rebuilds its captured graphs
+  g->noalloc = false;
+  if (rc == 100) return fail(GST_ERR_OOM, "worklist overflow in the allocation-free path");
+  if (rc) return rc;
+  *out_cost = out.cost[0];
+  if (out_status) *out_status = out.status[0];
+  const int32_t ne = (int32_t)out.uv[0].size();
+  if (out_num_edges) *out_num_edges = ne;
+  if (out_edge_uv)
+    for (int32_t i = 0; i < std::min(ne, edge_cap); ++i) {
+      out_edge_uv[2 * i] = out.uv[0][i].first;
+      out_edge_uv[2 * i + 1] = out.uv[0][i].second;
+    }
+  local.seconds_total =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  local.bytes_alg = 16 * local.states_expanded + 12 * local.grow_relax + 8 * local.merge_relax +
+                    12 * local.improvements + 4 * local.lb_terms;
+  if (st) *st = local;
+  if (out_edge_uv && ne > edge_cap)
+    return fail(GST_ERR_CAPACITY, "edge_cap too small; *out_num_edges holds the tree size");
+  return GST_OK;
 }
 
 int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* d_ids,

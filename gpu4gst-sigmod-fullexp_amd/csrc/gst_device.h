@@ -214,5 +214,8 @@ void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* ne
 void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
                             hipStream_t st);
 void launch_band_start(const DevCtx& c, hipStream_t st);
+// This GPU's address of the allocation-free path's scratch array (resolved
+// once per device with hipGetSymbolAddress; the current device must be set).
+hipError_t noalloc_scratch(void** out);
 
 }  // namespace gst

@@ -26,6 +26,10 @@
 // coalesced edge loop.
 #include "gst_device.h"
 
+#include <mutex>
+
+#include "../../include/gst.h"
+
 namespace gst {
 
 namespace {
@@ -1969,6 +1973,33 @@ void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float
   if (nq <= 0) return;
   int grid = nq < 4096 ? nq : 4096;
   hipLaunchKernelGGL(k_backward, grid, 256, 0, st, nq, off, ids, gout, gw, nund);
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (gst_solve_noalloc): every
+// per-query device buffer of that path is carved from this array. Each GPU
+// holds its own copy from module load until the process ends. Host code
+// never passes the name itself to a kernel or copy: noalloc_scratch()
+// resolves the current GPU's address once and keeps it per device.
+static __device__ __align__(256) unsigned char scratch[GST_NOALLOC_SCRATCH_BYTES];
+
+hipError_t noalloc_scratch(void** out) {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* addr[kMaxDevices] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidValue;
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return hipSuccess;
 }
 
 }  // namespace gst

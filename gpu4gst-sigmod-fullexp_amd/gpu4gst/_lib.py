@@ -14,7 +14,7 @@ EXPORTS = ("gst_graph_create", "gst_graph_set_edge_ids", "gst_graph_set_weights"
            "gst_graph_update_weights",
            "gst_groups_set", "gst_set_option", "gst_solve", "gst_solve_batch", "gst_backward",
            "gst_query_footprint", "gst_last_error", "gst_graph_destroy", "gst_version",
-           "gst_autotune", "gst_trace_get")
+           "gst_autotune", "gst_trace_get", "gst_solve_noalloc", "gst_noalloc_bytes")
 
 OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_INTERNAL = 0, 1, 2, 3, 4, 5
 OPT = dict(chunk=1, grid=2, iters_per_sync=3, hop_prune=4, profile=5, infeasible_cost=6,
@@ -86,5 +86,8 @@ def lib():
         L.gst_graph_destroy.argtypes = [vp]
         L.gst_autotune.argtypes = [vp, vp, vp, vp, i32, i32, C.c_double, vp]
         L.gst_trace_get.argtypes = [vp, vp, i64, C.POINTER(i64)]
+        L.gst_solve_noalloc.argtypes = [vp, vp, i32, i32, vp, vp, i32, vp, vp, vp]
+        L.gst_noalloc_bytes.restype = i64
+        L.gst_noalloc_bytes.argtypes = [vp, i32, i32]
         _LIB = L
     return _LIB

@@ -195,6 +195,27 @@ int gst_solve_batch(gst_graph* g, const int64_t* q_offsets, const int32_t* q_gro
                     int32_t* out_status, int64_t* out_edge_offsets, int32_t* out_edge_uv,
                     int32_t* out_edge_ids, int64_t edge_cap, gst_stats* out_stats);
 
+/* gst_solve without device allocation: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). No counterpart in the reference. Every per-query device buffer
+ * (state table, stamps, one-label rows, group masks, route tables,
+ * worklists, counters of the call, tree output) is carved from a file-scope
+ * device array of GST_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+ * ends; the graph and its groups stay where gst_graph_create put them. A
+ * query that needs more scratch is refused with GST_ERR_OOM (the message
+ * gives both sizes; gst_noalloc_bytes tells in advance). Trees longer than
+ * GST_OPT_MAX_TREE_EDGES are not recovered again (status GST_Q_NO_TREE, exact
+ * cost); stored backpointers and captured graphs are not used. All calls on
+ * one GPU share its array: calls that overlap on one GPU (from several
+ * threads or streams) race and must be serialised by the caller. */
+#define GST_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int gst_solve_noalloc(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D,
+                      int32_t* out_cost, int32_t* out_edge_uv, int32_t edge_cap,
+                      int32_t* out_num_edges, int32_t* out_status, gst_stats* out_stats);
+/* Scratch bytes gst_solve_noalloc needs for one query of k groups and hop
+ * bound D on this graph (with the one-query settings in force). */
+int64_t gst_noalloc_bytes(gst_graph* g, int32_t k, int32_t D);
+
 /* Backward of cost_q with respect to the undirected edge weights: device
  * pointers only. grad_w[e] += sum over queries q of grad_out[q] for every tree
  * edge e of query q (tree q = edge_ids[edge_offsets[q] .. edge_offsets[q+1])).

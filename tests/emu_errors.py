@@ -183,6 +183,32 @@ def main():
     except GSTError:
         check("tune_bad_batch", True)
     G.close()
+    # 7. the allocation-free entry point: two calls in a row on different
+    # graphs, the second smaller, agree with gst_solve (the scratch array keeps
+    # the first call's contents)
+    import ctypes as C
+    from gpu4gst import _lib
+    L = _lib.lib()
+    big, bgrp = workloads.small(120, 300, 8, seed_=11, whi=50, member_p=0.04)
+    small, sgrp = workloads.small(30, 60, 4, seed_=12, whi=9, member_p=0.1)
+    for gg, gp, q in ((big, bgrp, [0, 2, 4, 5, 7]), (small, sgrp, [1, 3])):
+        Gx = GSTGraph.from_generated(gg)
+        Gx.set_groups(gp)
+        for D in (-1, 3):
+            qa = np.asarray(q, np.int32)
+            cost, ne, st = C.c_int32(0), C.c_int32(0), C.c_int32(0)
+            uv = np.zeros(2 * 512, np.int32)
+            rc = L.gst_solve_noalloc(Gx._h, qa.ctypes.data_as(C.c_void_p), len(q), D,
+                                     C.byref(cost), uv.ctypes.data_as(C.c_void_p), 512,
+                                     C.byref(ne), C.byref(st), None)
+            ref = Gx.solve(qa, D=D)
+            ok = rc == 0 and (cost.value, st.value) == (ref.cost, ref.status)
+            if ok and st.value == 0 and ne.value:
+                code, wsum = O.verify_tree(gg, gp, qa, uv[:2 * ne.value].reshape(-1, 2), D)
+                ok = code == 0 and wsum == cost.value
+            check("noalloc", ok, f"n={gg.n} q={q} D={D} rc={rc} {cost.value} {ref.cost}")
+        Gx.close()
+
     print(json.dumps({"cases": cases, "fails": fails}))
 
 

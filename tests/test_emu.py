@@ -45,7 +45,7 @@ def test_emulated_error_and_range_handling():
                        env=env, capture_output=True, text=True, timeout=900)
     assert r.returncode == 0, r.stderr[-3000:]
     out = json.loads(r.stdout.strip().splitlines()[-1])
-    assert out["cases"] == 65
+    assert out["cases"] == 69
     assert out["fails"] == [], out["fails"]
 
 
