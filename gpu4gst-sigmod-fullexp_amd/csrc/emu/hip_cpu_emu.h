@@ -46,7 +46,7 @@ inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 #define __shared__ static
 #define __launch_bounds__(...)
 #define __forceinline__ inline
-#define __align__(n) alignas(n)
+#define __align__(n) __attribute__((aligned(n)))
 #define __HIP_MEMORY_SCOPE_AGENT 0
 
 typedef int hipError_t;

@@ -152,7 +152,10 @@ struct gst_graph {
   int64_t eff_key = -1;        // the effective settings baked into the captured graphs
   int32_t maxw = 1;
   // engine buffers (grow-only)
-  DBuf T, Es, BPs, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, cnt, iter, ctr, sched;
+  // wl0 / wl1: the frontier lists (DP: vertex items); wls: the DP's unfolded
+  // state items of the current iteration; Ms: DP pending masks
+  DBuf T, Es, BPs, Ms, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, wls, cnt, iter,
+      ctr, sched;
   DBuf out_uv, out_eid, out_n, out_err;
   uint32_t* h_cnt = nullptr;  // pinned
   int32_t h_nslots[1] = {0};  // slots of the batch being solved
@@ -188,14 +191,14 @@ struct gst_graph {
   // they count as free when the next batch is admitted
   size_t held_bytes() const {
     size_t s = 0;
-    for (const DBuf* b : {&T, &Es, &BPs, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
+    for (const DBuf* b : {&T, &Es, &BPs, &Ms, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &wls, &out_uv,
                           &out_eid})
       s += b->bytes;
     return s;
   }
   void release_batch_buffers() {
     drop_graphs();
-    for (DBuf* b : {&T, &Es, &BPs, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &out_uv,
+    for (DBuf* b : {&T, &Es, &BPs, &Ms, &GMs, &far0, &far1, &C, &R, &VVb, &wl0, &wl1, &wls, &out_uv,
                     &out_eid})
       b->release();
     wl_cap = 0;
@@ -218,10 +221,20 @@ constexpr int kAutoBandSlots = 32;  // automatic DP bands from this many plain q
 constexpr int kAutoBandDiv = 256;   // first band width = upper bound / 256 (>= 1), doubling
 constexpr int kAutoSsspBand = 16;   // first one-label band: labels below 16, doubling per band
 
-// Table footprint of one slot (int32 elements) and its worklist share.
+// Pending-mask words per (vertex, depth) of a query of k groups.
+int32_t mask_words(int32_t k) { return k <= 5 ? 1 : (1 << k) / 32; }
+
+// Pending-mask words of one slot: one mask per (vertex, depth).
+int64_t slot_mask_words(const gst_graph* g, int32_t k, int32_t D) {
+  return (int64_t)(D < 0 ? 1 : D + 1) * g->Vp * mask_words(k);
+}
+
+// Table footprint of one slot (int32 elements): state words, one-label far
+// keys, pending masks, group masks, one-label distances, route tables.
 int64_t slot_ints(const gst_graph* g, int32_t k, int32_t D) {
   const int64_t rows = rows_of(k, D);
-  return 2 * rows * g->Vp + rows * g->Vp + g->Vp + (int64_t)g->Vp * CROW + RTABLE;
+  return 2 * rows * g->Vp + g->Vp + slot_mask_words(g, k, D) + g->Vp +
+         (int64_t)g->Vp * CROW + RTABLE;
 }
 
 int64_t slot_items(gst_graph* g, int32_t k, int32_t D) { return (int64_t)rows_of(k, D) * g->nch(); }
@@ -241,6 +254,7 @@ DevCtx make_ctx(gst_graph* g) {
   c.hop_prune = g->hop_prune;
   c.T = g->T.as<int32_t>();
   c.E = g->Es.as<int32_t>();
+  c.M = g->Ms.as<uint32_t>();
   c.GM = g->GMs.as<uint32_t>();
   c.C = g->C.as<int32_t>();
   c.R = g->R.as<int32_t>();
@@ -308,17 +322,37 @@ void select_settings(gst_graph* g, int64_t nq) {
   }
 }
 
+// Counters of the lists (uint32 each): the two frontier lists, the two far
+// lists, the DP's unfolded state list.
+constexpr int kCntStates = 4;
+constexpr int kCntWords = 5;
+
+// The expand step of half-iteration h (list h -> list 1 - h). The DP first
+// unfolds the vertex items of list h into the state list.
+void launch_expand_step(gst_graph* g, const DevCtx& c, int mode, int h) {
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  uint64_t* l[2] = {g->wl0.as<uint64_t>(), g->wl1.as<uint64_t>()};
+  const int grid = expand_grid(g);
+  if (mode == 0) {
+    launch_sssp_expand(c, WL{l[h], cnt + h, l[1 - h], cnt + (1 - h)}, grid, g->stream);
+    return;
+  }
+  launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, grid, g->stream);
+  launch_dp_expand(c, WL{g->wls.as<uint64_t>(), cnt + kCntStates, l[1 - h], cnt + (1 - h)}, grid,
+                   mode == 2, g->stream);
+}
+
+void launch_advance_step(gst_graph* g, const DevCtx& c, int mode, int h) {
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  launch_advance(c, cnt + h, cnt + (1 - h), mode != 0, mode != 0 ? cnt + kCntStates : nullptr,
+                 g->stream);
+}
+
 // One frontier iteration pair: list 0 -> list 1, list 1 -> list 0.
 void launch_pair(gst_graph* g, const DevCtx& c, int mode) {
-  uint32_t* cnt = g->cnt.as<uint32_t>();
-  WL a{g->wl0.as<uint64_t>(), cnt, g->wl1.as<uint64_t>(), cnt + 1};
-  WL b{g->wl1.as<uint64_t>(), cnt + 1, g->wl0.as<uint64_t>(), cnt};
-  const int grid = expand_grid(g);
   for (int h = 0; h < 2; ++h) {
-    const WL& w = h ? b : a;
-    if (mode == 0) launch_sssp_expand(c, w, grid, g->stream);
-    else launch_dp_expand(c, w, grid, mode == 2, g->stream);
-    launch_advance(c, cnt + h, cnt + (1 - h), mode != 0, g->stream);
+    launch_expand_step(g, c, mode, h);
+    launch_advance_step(g, c, mode, h);
   }
 }
 
@@ -333,17 +367,26 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
   while (true) {
     if (g->profile) {
       const int grid = expand_grid(g);
-      WL a{g->wl0.as<uint64_t>(), cnt, g->wl1.as<uint64_t>(), cnt + 1};
-      WL b{g->wl1.as<uint64_t>(), cnt + 1, g->wl0.as<uint64_t>(), cnt};
+      uint64_t* l[2] = {g->wl0.as<uint64_t>(), g->wl1.as<uint64_t>()};
       for (int h = 0; h < 2; ++h) {
+        // the DP's unfold step runs before the timed expand launch
+        if (mode != 0)
+          launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, grid,
+                           g->stream);
         uint32_t items = 0;
-        if (g->trace)
-          HIP_TRY(hipMemcpy(&items, cnt + h, sizeof(items), hipMemcpyDeviceToHost));
+        if (g->trace) {
+          HIP_TRY(hipStreamSynchronize(g->stream));
+          HIP_TRY(hipMemcpy(&items, cnt + (mode != 0 ? kCntStates : h), sizeof(items),
+                            hipMemcpyDeviceToHost));
+        }
         HIP_TRY(hipEventRecord(g->ev0, g->stream));
-        if (mode == 0) launch_sssp_expand(c, h ? b : a, grid, g->stream);
-        else launch_dp_expand(c, h ? b : a, grid, mode == 2, g->stream);
+        if (mode == 0)
+          launch_sssp_expand(c, WL{l[h], cnt + h, l[1 - h], cnt + (1 - h)}, grid, g->stream);
+        else
+          launch_dp_expand(c, WL{g->wls.as<uint64_t>(), cnt + kCntStates, l[1 - h], cnt + (1 - h)},
+                           grid, mode == 2, g->stream);
         HIP_TRY(hipEventRecord(g->ev1, g->stream));
-        launch_advance(c, cnt + h, cnt + (1 - h), mode != 0, g->stream);
+        launch_advance_step(g, c, mode, h);
         HIP_TRY(hipEventSynchronize(g->ev1));
         float ms = 0;
         HIP_TRY(hipEventElapsedTime(&ms, g->ev0, g->ev1));
@@ -437,7 +480,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   const int nslots = (int)qs.size();
   std::vector<SlotDesc> sd(nslots);
   std::vector<Task> tasks;
-  int64_t t_tot = 0, e_tot = 0, items = 0, sssp_items = 0;
+  int64_t t_tot = 0, e_tot = 0, m_tot = 0, items = 0, sssp_items = 0;
   for (int s = 0; s < nslots; ++s) {
     const QueryIn& q = qs[s];
     SlotDesc d;
@@ -453,8 +496,11 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     d.nsub = 1 << q.k;
     d.vv_off = s * VV;
     d.nrows = (int32_t)rows;
+    d.mw = mask_words(q.k);
+    d.m_off = m_tot;
     t_tot += rows * g->Vp;
-    e_tot += rows * g->Vp;
+    e_tot += g->Vp;
+    m_tot += slot_mask_words(g, q.k, q.D);
     items += slot_items(g, q.k, q.D);
     sssp_items += g->nch();  // one-label items are (slot, vertex[, chunk])
     sd[s] = d;
@@ -471,7 +517,8 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   grew |= e_grew;
   // backpointer words need no initialisation: every state the tree recovery
   // visits was written in this batch, and each word is checked before use
-  if (g->bp_call) HIP_TRY(g->BPs.ensure((size_t)e_tot * 4, &grew));
+  if (g->bp_call) HIP_TRY(g->BPs.ensure((size_t)t_tot * 4, &grew));
+  HIP_TRY(g->Ms.ensure((size_t)m_tot * 4, &grew));
   HIP_TRY(g->C.ensure((size_t)nslots * g->Vp * CROW * 4, &grew));
   HIP_TRY(g->GMs.ensure((size_t)nslots * g->Vp * 4, &grew));
   HIP_TRY(g->R.ensure((size_t)nslots * RTABLE * 4, &grew));
@@ -480,9 +527,11 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(g->slots.ensure((size_t)nslots * sizeof(SlotDesc), &grew));
   HIP_TRY(g->sched.ensure((size_t)nslots * sizeof(Sched), &grew));
   HIP_TRY(g->tasks.ensure(tasks.size() * sizeof(Task) + 16, &grew));
-  if ((uint64_t)cap * 8 > g->wl0.bytes || (uint64_t)cap * 8 > g->wl1.bytes) {
+  if ((uint64_t)cap * 8 > g->wl0.bytes || (uint64_t)cap * 8 > g->wl1.bytes ||
+      (uint64_t)cap * 8 > g->wls.bytes) {
     HIP_TRY(g->wl0.ensure((size_t)cap * 8, &grew));
     HIP_TRY(g->wl1.ensure((size_t)cap * 8, &grew));
+    HIP_TRY(g->wls.ensure((size_t)cap * 8, &grew));
   }
   // A* bands (near/far lists) pay off when many queries share each iteration;
   // a lone query runs unordered (measured on the 168k-vertex workload with the
@@ -513,7 +562,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     HIP_TRY(g->far0.ensure((size_t)cap * 8, &grew));
     HIP_TRY(g->far1.ensure((size_t)cap * 8, &grew));
   }
-  size_t list_bytes = std::min(g->wl0.bytes, g->wl1.bytes);
+  size_t list_bytes = std::min({g->wl0.bytes, g->wl1.bytes, g->wls.bytes});
   if (far_lists) list_bytes = std::min({list_bytes, g->far0.bytes, g->far1.bytes});
   const uint32_t new_cap = (uint32_t)std::min<size_t>(list_bytes / 8, UINT32_MAX - 65536);
   if (new_cap != g->wl_cap) {
@@ -553,7 +602,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   ++g->batch_no;
   HIP_TRY(hipEventRecord(g->ph[0], g->stream));
   launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
-  launch_fill_i32(g->cnt.as<int32_t>(), 2, 0, g->stream);
+  launch_fill_i32(g->cnt.as<int32_t>(), kCntWords, 0, g->stream);
   launch_init_slot_tables(c, nslots, (int64_t)nslots * g->Vp * CROW, g->stream);
   const Task* dt = g->tasks.as<Task>();
   const int nt = (int)tasks.size();
@@ -583,6 +632,8 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
   launch_route(c, nslots, g->stream);
   launch_lb_table(c, nslots, g->stream);
+  // pending masks start empty (an overflowed earlier run may have left bits)
+  launch_fill_i32(reinterpret_cast<int32_t*>(g->Ms.p), m_tot, 0, g->stream);
   launch_dp_seed(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(),
                  cnt, g->stream);
   launch_dp_sched_init(c, nslots, g->stream);
@@ -1008,14 +1059,14 @@ int gst_graph_create(const void* beg_pos, const void* csr, const void* weight, i
     gst_graph_destroy(g);
     return fail(GST_ERR_DEVICE, "stream / event / pinned allocation failed");
   }
-  if (g->cnt.ensure(16) != hipSuccess || g->iter.ensure(16) != hipSuccess ||
+  if (g->cnt.ensure(kCntWords * 4) != hipSuccess || g->iter.ensure(16) != hipSuccess ||
       g->ctr.ensure(C_NUM * 8) != hipSuccess) {
     gst_graph_destroy(g);
     return fail(GST_ERR_OOM, "counter allocation failed");
   }
   int32_t one[4] = {1, 0, 0, 0};
   if (hipMemcpy(g->iter.p, one, 16, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemset(g->cnt.p, 0, 16) != hipSuccess) {
+      hipMemset(g->cnt.p, 0, kCntWords * 4) != hipSuccess) {
     gst_graph_destroy(g);
     return fail(GST_ERR_DEVICE, "counter initialisation failed");
   }
@@ -1187,8 +1238,8 @@ int64_t gst_query_footprint(gst_graph* g, int32_t k, int32_t D) {
   if (!g || k < 1 || k > GST_MAX_K) return -1;
   const double wl_items =
       std::max(kWorklistFraction * (double)slot_items(g, k, D), (double)g->nch());
-  // two worklists and the two far lists of the banded schedules
-  return slot_ints(g, k, D) * 4 + (int64_t)(4 * 8 * wl_items) +
+  // two frontier lists, the unfolded state list and the two far lists
+  return slot_ints(g, k, D) * 4 + (int64_t)(5 * 8 * wl_items) +
          (int64_t)g->max_tree_edges * 12 + 1024;
 }
 
@@ -1429,13 +1480,17 @@ struct Carve {
   }
 };
 
+constexpr int kNoallocBufs = 20;
+
 // The buffers of one slot at full worklist capacity (lists never overflow).
 void carve_slot(gst_graph* g, int32_t k, int32_t D, DBuf* bufs[], Carve& cv) {
   const int64_t rows = rows_of(k, D);
   const int64_t items = slot_items(g, k, D);
   const int64_t cap = std::max<int64_t>(items, g->nch()) + 4096;
   const size_t sizes[] = {(size_t)rows * g->Vp * 8,        // T
-                          (size_t)rows * g->Vp * 4,        // Es
+                          (size_t)g->Vp * 4,               // Es
+                          (size_t)slot_mask_words(g, k, D) * 4,  // Ms
+                          (size_t)cap * 8,                 // wls
                           (size_t)g->Vp * CROW * 4,        // C
                           (size_t)g->Vp * 4,               // GMs
                           (size_t)RTABLE * 4,              // R
@@ -1448,14 +1503,14 @@ void carve_slot(gst_graph* g, int32_t k, int32_t D, DBuf* bufs[], Carve& cv) {
                           (size_t)g->max_tree_edges * 8,   // out_uv
                           (size_t)g->max_tree_edges * 4,   // out_eid
                           4, 4};                           // out_n, out_err
-  for (int i = 0; i < 18; ++i) cv.take(*bufs[i], sizes[i]);
+  for (int i = 0; i < kNoallocBufs; ++i) cv.take(*bufs[i], sizes[i]);
 }
 
-DBuf* const* slot_bufs(gst_graph* g, DBuf* out[18]) {
-  DBuf* b[18] = {&g->T,    &g->Es,   &g->C,     &g->GMs,  &g->R,    &g->VVb,
+DBuf* const* slot_bufs(gst_graph* g, DBuf* out[kNoallocBufs]) {
+  DBuf* b[kNoallocBufs] = {&g->T, &g->Es, &g->Ms, &g->wls, &g->C, &g->GMs, &g->R, &g->VVb,
                  &g->bestw, &g->slots, &g->sched, &g->tasks, &g->wl0,  &g->wl1,
                  &g->far0,  &g->far1,  &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
-  for (int i = 0; i < 18; ++i) out[i] = b[i];
+  for (int i = 0; i < kNoallocBufs; ++i) out[i] = b[i];
   return out;
 }
 
@@ -1469,9 +1524,9 @@ int64_t gst_noalloc_bytes(gst_graph* g, int32_t k, int32_t D) {
   g->noalloc = true;
   select_settings(g, 1);
   g->noalloc = save;
-  DBuf tmp[18];
-  DBuf* bufs[18];
-  for (int i = 0; i < 18; ++i) bufs[i] = &tmp[i];
+  DBuf tmp[kNoallocBufs];
+  DBuf* bufs[kNoallocBufs];
+  for (int i = 0; i < kNoallocBufs; ++i) bufs[i] = &tmp[i];
   Carve cv{nullptr};
   carve_slot(g, k, D < 0 ? -1 : D, bufs, cv);
   return (int64_t)cv.used;
@@ -1492,13 +1547,13 @@ int gst_solve_noalloc(gst_graph* g, const int32_t* query_groups, int32_t k, int3
   HIP_TRY(noalloc_scratch(&base));
   g->noalloc = true;
   select_settings(g, 1);
-  DBuf* bufs[18];
+  DBuf* bufs[kNoallocBufs];
   slot_bufs(g, bufs);
   Carve measure{nullptr};
   {
-    DBuf tmp[18];
-    DBuf* tb[18];
-    for (int i = 0; i < 18; ++i) tb[i] = &tmp[i];
+    DBuf tmp[kNoallocBufs];
+    DBuf* tb[kNoallocBufs];
+    for (int i = 0; i < kNoallocBufs; ++i) tb[i] = &tmp[i];
     carve_slot(g, qs[0].k, qs[0].D, tb, measure);
   }
   if (measure.used > (size_t)GST_NOALLOC_SCRATCH_BYTES) {
@@ -1508,8 +1563,8 @@ int gst_solve_noalloc(gst_graph* g, const int32_t* query_groups, int32_t k, int3
                                  std::to_string((unsigned long long)GST_NOALLOC_SCRATCH_BYTES));
   }
   // the graph's own buffers step aside for the call and come back unchanged
-  DBuf saved[18];
-  for (int i = 0; i < 18; ++i) {
+  DBuf saved[kNoallocBufs];
+  for (int i = 0; i < kNoallocBufs; ++i) {
     saved[i] = *bufs[i];
     *bufs[i] = DBuf();
   }
@@ -1531,7 +1586,7 @@ int gst_solve_noalloc(gst_graph* g, const int32_t* query_groups, int32_t k, int3
   gst_stats local;
   std::memset(&local, 0, sizeof(local));
   rc = solve_lockstep(g, qs, 1.0, out, &local);
-  for (int i = 0; i < 18; ++i) *bufs[i] = saved[i];
+  for (int i = 0; i < kNoallocBufs; ++i) *bufs[i] = saved[i];
   g->use_graphs = save_graphs;
   g->bp_call = save_bp;
   g->wl_cap = save_cap;
@@ -1573,7 +1628,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* 
 void gst_graph_destroy(gst_graph* g) {
   if (!g) return;
   g->drop_graphs();
-  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->GMs, &g->far0, &g->far1,
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->Ms, &g->wls, &g->GMs, &g->far0, &g->far1,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
                   &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();

@@ -26,6 +26,11 @@ constexpr int VV = 64;              // 8x8 group-to-group distances per slot
 //   bits  0..23  vertex v        bits 24..31  group set p (one-label phase: group index)
 //   bits 32..35  hop layer d     bits 36..51  chunk index of v's CSR row
 //   bits 52..63  query slot
+// The DP frontier alternates two lists: vertex items (p = 0, chunk 0: vertex v
+// at depth d has pending states, listed in its pending mask) and state items
+// (one per pending state and chunk), made from the vertex items by
+// k_dp_unfold at the start of every iteration with the states of one vertex
+// and chunk next to each other.
 __host__ __device__ inline uint64_t mk_item(uint32_t v, uint32_t p, uint32_t d, uint32_t ch,
                                             uint32_t s) {
   return (uint64_t)v | ((uint64_t)p << 24) | ((uint64_t)d << 32) | ((uint64_t)ch << 36) |
@@ -50,27 +55,28 @@ __host__ __device__ inline uint32_t wit_onelabel(uint32_t v, uint32_t p) {
   return 0x80000000u | (p << 24) | v;
 }
 
-// Per-query-slot layout; all offsets count int32 elements of their arena.
+// Per-query-slot layout; all offsets count elements of their arena.
 struct SlotDesc {
   int64_t t_off;   // state words, vertex-major: entry v * nrows + d * nsub + p holds
                    // {cost, completion bound} as two int32, so all states of one vertex
                    // (every merge partner) share a few cache lines; row 0 (p = 0) is
-                   // no state and never read
-  int64_t e_off;   // enqueue stamps of DP states (iteration numbers), same indexing as T
+                   // no state and never read. Stored backpointers use the same indices.
+  int64_t e_off;   // one int32 per vertex: the one-label phase's far-list key
   int64_t c_off;   // one-label distances c[v * 8 + i]
   int64_t r_off;   // route tables [p][72]
   int32_t k, full, D, nsub;  // D < 0 = plain problem
   int32_t vv_off;  // group distance matrix [8][8] in the vv array
   int32_t nrows;   // rows per vertex: nsub (plain) or nsub * (D + 1) (hop)
-  int32_t pad[2];
+  int32_t mw;      // pending-mask words per (vertex, depth): nsub / 32 rounded up
+  int32_t pad;
+  int64_t m_off;   // DP pending masks: word m_off + (d * Vp + v) * mw + p / 32, bit p % 32,
+                   // says state (v, p, d) waits for the next iteration; bit 0 of word 0
+                   // (p = 0 is no state) says the vertex item (v, d) is queued
 };
 
-// Index (in int2 entries of T, or int32 entries of E) of row `row` of vertex v.
+// Index (in int2 entries of T, or int32 entries of BP) of row `row` of vertex v.
 __host__ __device__ inline int64_t t_ix(const SlotDesc& S, int64_t row, int64_t v) {
   return S.t_off + v * S.nrows + row;
-}
-__host__ __device__ inline int64_t e_ix(const SlotDesc& S, int64_t row, int64_t v) {
-  return S.e_off + v * S.nrows + row;
 }
 
 // Counters accumulated on device (uint64 each).
@@ -122,6 +128,7 @@ struct DevCtx {
   int32_t hop_prune;
   int32_t* T;
   int32_t* E;
+  uint32_t* M;                   // DP pending masks (SlotDesc::m_off)
   uint32_t* GM;                  // query-group mask of each vertex [slot][Vp]
   int32_t* C;
   int32_t* R;
@@ -192,7 +199,10 @@ void launch_groups_setup(const DevCtx& c, const Task* tasks, int ntasks, const i
 void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st);
 void launch_dp_expand(const DevCtx& c, const WL& w, int grid, bool hop, hipStream_t st);
 void launch_advance(const DevCtx& c, uint32_t* cnt_consumed, const uint32_t* cnt_next, bool dp,
-                    hipStream_t st);
+                    uint32_t* cnt_states, hipStream_t st);
+// Vertex items of the DP frontier -> one state item per pending state and chunk.
+void launch_dp_unfold(const DevCtx& c, const uint64_t* vin, const uint32_t* vin_cnt,
+                      uint64_t* sout, uint32_t* sout_cnt, int grid, hipStream_t st);
 void launch_vv(const DevCtx& c, const Task* tasks, int ntasks, const int64_t* goff,
                const int32_t* gmem, hipStream_t st);
 void launch_route(const DevCtx& c, int nslots, hipStream_t st);

@@ -387,6 +387,39 @@ __device__ inline uint32_t pending_labels(uint32_t w, uint32_t it) {
   return (w >> 8) == it ? (w & 0xffu) : 0xffu;
 }
 
+// Marks DP state (v, p, d) of slot S pending for the next iteration: sets
+// bit p of the vertex's pending mask, and bit 0 of its first word, the
+// "vertex queued" flag. Returns true when this call raised the flag: the
+// caller then appends the vertex item (v, d) once. A state already pending
+// costs one atomic and queues nothing. k_dp_unfold clears the mask when it
+// lists the states, before any thread of the next iteration marks again.
+__device__ inline bool mark_state(const DevCtx& c, const SlotDesc& S, uint32_t v, uint32_t p,
+                                  uint32_t d) {
+  uint32_t* m = c.M + S.m_off + ((int64_t)d * c.Vp + v) * S.mw;
+  const uint32_t wi = p >> 5, bit = 1u << (p & 31u);
+  if (wi == 0) return (atomicOr(m, bit | 1u) & 1u) == 0;
+  if (atomicOr(m + wi, bit) & bit) return false;  // already pending: its vertex is queued
+  return (atomicOr(m, 1u) & 1u) == 0;
+}
+
+// Plain slots: T(v, full) <- t + T(v, full ^ p), the complement merge a state
+// gets when it is expanded. A state that will never be expanded (dropped by
+// the 1/2 rule, or not queued under the 2/3 rule) gets it when it is dropped
+// instead, so two complementary states that both exist are always combined,
+// in whatever order the frontier reaches them. (States dropped by their
+// completion bound need none: t + lb > best + 2 bounds every such merge.)
+__device__ inline void complement_merge(const DevCtx& c, const SlotDesc& S, uint32_t s,
+                                        uint32_t v, uint32_t p, int32_t t) {
+  const int32_t t2 = c.T[2 * t_ix(S, (uint32_t)S.full ^ p, v)];
+  if (t2 >= INF || t >= INF) return;
+  const int32_t mcost = t + t2;
+  int32_t* dfull = &c.T[2 * t_ix(S, S.full, v)];
+  if (mcost < *dfull && mcost < atomicMin(dfull, mcost)) {
+    if (c.BP) c.BP[t_ix(S, S.full, v)] = (int32_t)bp_merge(p, 0, 0);
+    ub_update(&c.bestw[s], mcost, wit_full(v, 0));
+  }
+}
+
 __device__ inline void flush(unsigned long long* ctr, int idx, unsigned long long v) {
   // wave-reduce, then one atomic per wave
 #pragma unroll
@@ -580,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       if (banded && eff < radius && lo_all < INF) {
         const int32_t cut_all = lo < INF ? re : row_cut_near(c.wt, rb0, rend, eff - 1 - lo_all);
         const bool more = cut_all < rend && c.wt[cut_all] <= radius - 1 - lo_all;
-        if ((beyond || more) && atomicMax(&c.E[e_ix(S, 0, v)], bkey) < bkey) {
+        if ((beyond || more) && atomicMax(&c.E[S.e_off + v], bkey) < bkey) {
           far_want = 1;
           far_item = mk_item(v, 0, 0, 0, s);
         }
@@ -710,7 +743,8 @@ void launch_sssp_expand(const DevCtx& c, const WL& w, int grid, hipStream_t st) 
 // Consumes one iteration: clears the consumed list's counter, advances the
 // iteration number, moves each slot's threshold past an exhausted band and
 // records frontier statistics.
-__global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_next, int dp) {
+__global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_next, int dp,
+                          uint32_t* cnt_states) {
   __shared__ int n_active;
   if (threadIdx.x == 0) n_active = 0;
   __syncthreads();
@@ -731,13 +765,73 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
     if (dp && produced > c.ctr[C_MAX_FRONTIER]) c.ctr[C_MAX_FRONTIER] = produced;
     if (c.trace) c.ctr[C_ACTIVE] = (unsigned long long)n_active;  // slots that expanded
     *cnt_consumed = 0;
+    if (cnt_states) *cnt_states = 0;  // the DP's unfolded state list, consumed too
     *c.iter += 1;
   }
 }
 
 void launch_advance(const DevCtx& c, uint32_t* cnt_consumed, const uint32_t* cnt_next, bool dp,
-                    hipStream_t st) {
-  hipLaunchKernelGGL(k_advance, 1, 256, 0, st, c, cnt_consumed, cnt_next, dp ? 1 : 0);
+                    uint32_t* cnt_states, hipStream_t st) {
+  hipLaunchKernelGGL(k_advance, 1, 256, 0, st, c, cnt_consumed, cnt_next, dp ? 1 : 0,
+                     cnt_states);
+}
+
+// The DP frontier of one iteration: every vertex item (v, d) of slot s turns
+// into one state item per pending state p and chunk of v's row, chunk-major
+// and state-minor, so that the states of one vertex and chunk are adjacent
+// in the list (the expand kernel's lanes then relax the same edges for them
+// together, reading each neighbour's state row once for several subsets).
+// The pending mask is cleared as it is read: the vertex was queued once
+// (mark_state), so this is its only reader, and the expand kernel that marks
+// states for the next iteration starts after this kernel ends.
+__global__ __launch_bounds__(BLOCK) void k_dp_unfold(DevCtx c, const uint64_t* vin,
+                                                     const uint32_t* vin_cnt, uint64_t* sout,
+                                                     uint32_t* sout_cnt) {
+  __shared__ int32_t s_off[BLOCK + 1];
+  __shared__ int32_t s_wsum[BLOCK / WAVE];
+  __shared__ uint32_t s_base;
+  const uint32_t n = min(*vin_cnt, c.cap);
+  const int tid = threadIdx.x;
+  for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
+    const uint32_t idx = tile * BLOCK + tid;
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+    uint32_t v = 0, d = 0, s = 0, nch = 0, mw = 0;
+    int32_t cnt = 0;
+    if (idx < n) {
+      const uint64_t it = vin[idx];
+      v = it_v(it);
+      d = it_d(it);
+      s = it_s(it);
+      const SlotDesc S = c.slots[s];
+      mw = (uint32_t)S.mw;
+      uint32_t* m = c.M + S.m_off + ((int64_t)d * c.Vp + v) * S.mw;
+      for (uint32_t i = 0; i < mw; ++i) {
+        w[i] = m[i];
+        m[i] = 0u;
+      }
+      w[0] &= ~1u;  // the queued flag is no state
+      const int32_t pc = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
+      nch = nchunks(c, v);
+      cnt = pc * (int32_t)nch;
+    }
+    block_scan(cnt, s_off, s_wsum);
+    if (tid == 0) s_base = s_off[BLOCK] ? atomicAdd(sout_cnt, (uint32_t)s_off[BLOCK]) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + (uint32_t)s_off[tid];
+    for (uint32_t ch = 0; ch < nch; ++ch)
+      for (uint32_t i = 0; i < mw; ++i)
+        for (uint32_t rest = w[i]; rest; rest &= rest - 1u, ++pos) {
+          const uint32_t p = 32u * i + (uint32_t)__builtin_ctz(rest);
+          if (pos < c.cap) sout[pos] = mk_item(v, p, d, ch, s);
+          else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+        }
+    __syncthreads();  // s_off / s_base are reused by the next tile
+  }
+}
+
+void launch_dp_unfold(const DevCtx& c, const uint64_t* vin, const uint32_t* vin_cnt,
+                      uint64_t* sout, uint32_t* sout_cnt, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_dp_unfold, grid, BLOCK, 0, st, c, vin, vin_cnt, sout, sout_cnt);
 }
 
 // After the one-label phase of a plain query: every vertex whose distance to
@@ -801,7 +895,10 @@ __device__ inline int64_t item_row(const SlotDesc& S, uint64_t it) {
   return S.D >= 0 ? (int64_t)it_d(it) * S.nsub + it_p(it) : (int64_t)it_p(it);
 }
 
-__device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f) {
+// *half_drop (may be null) is set when a plain state is dropped by the 1/2
+// rule alone: the caller owes it its complement merge.
+__device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f,
+                                bool* half_drop = nullptr) {
   const SlotDesc S = c.slots[it_s(it)];
   const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
   const int32_t t = tl.x;
@@ -809,7 +906,10 @@ __device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f) {
   const int32_t best = best_of(&c.bestw[it_s(it)]);
   if (S.D >= 0) {
     if (c.hop_prune && best < INF && t + tl.y > best) return false;
-  } else if (t + tl.y > best + 2 || !(t < best / 2 + 2)) {
+  } else if (t + tl.y > best + 2) {
+    return false;
+  } else if (!(t < best / 2 + 2)) {
+    if (half_drop) *half_drop = true;
     return false;
   }
   *f = min(t + tl.y, INF - 1);
@@ -857,7 +957,6 @@ __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, ui
   const uint64_t* far = c.far[sel];
   uint64_t* far_out = c.far[1 - sel];
   uint32_t* far_out_cnt = c.far_cnt + (1 - sel);
-  const int32_t stamp = *c.iter;  // the iteration that will read the near list
   for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
     const uint32_t i = base + threadIdx.x;
     uint32_t want = 0, fwant = 0;
@@ -865,14 +964,21 @@ __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, ui
     if (i < n) {
       const uint64_t it = far[i];
       int32_t f;
-      if (far_live(c, it, &f)) {
+      bool half_drop = false;
+      const bool live = far_live(c, it, &f, &half_drop);
+      if (half_drop) {  // never expanded now: combine it with its complement first
+        const SlotDesc S = c.slots[it_s(it)];
+        complement_merge(c, S, it_s(it), it_v(it), it_p(it),
+                         c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
+      }
+      if (live) {
         const uint32_t v = it_v(it), s = it_s(it);
         const SlotDesc S = c.slots[s];
         if (f < c.sched[s].thr) {
-          // duplicates of a state in the far list collapse here
-          if (atomicMax(&c.E[e_ix(S, item_row(S, it), v)], stamp) < stamp) {
-            want = nchunks(c, v);
-            item = it;
+          // back to the frontier; duplicates of a state in the far list collapse here
+          if (mark_state(c, S, v, it_p(it), it_d(it))) {
+            want = 1;
+            item = mk_item(v, 0, it_d(it), 0, s);
           }
         } else {
           fwant = 1;
@@ -1187,7 +1293,6 @@ void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st) {
 __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, int ntasks,
                                                    const int64_t* goff, const int32_t* gmem,
                                                    uint64_t* wl, uint32_t* cnt) {
-  const int32_t stamp = *c.iter;
   for (int t = blockIdx.x; t < ntasks; t += gridDim.x) {
     const Task tk = tasks[t];
     const SlotDesc S = c.slots[tk.slot];
@@ -1208,7 +1313,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
         uint64_t item = 0;
         if (live) {
           atomicMin(&c.T[2 * t_ix(S, p, v)], 0);
-          if (c.BP) c.BP[e_ix(S, p, v)] = (int32_t)BP_BASE;
+          if (c.BP) c.BP[t_ix(S, p, v)] = (int32_t)BP_BASE;
           if ((int32_t)p == S.full) {
             ub_update(&c.bestw[tk.slot], 0, wit_full(v, 0));
           } else {
@@ -1217,9 +1322,9 @@ __global__ __launch_bounds__(BLOCK) void k_dp_seed(DevCtx c, const Task* tasks, 
                                          c.sched[tk.slot].R);
               if (sumc < INF) ub_update(&c.bestw[tk.slot], sumc, wit_onelabel(v, p));
             }
-            if (atomicMax(&c.E[e_ix(S, p, v)], stamp) < stamp) {
-              want = nchunks(c, v);
-              item = mk_item(v, p, 0, 0, (uint32_t)tk.slot);
+            if (mark_state(c, S, v, p, 0)) {
+              want = 1;
+              item = mk_item(v, 0, 0, 0, (uint32_t)tk.slot);
             }
           }
           p = (p - 1) & gsv;
@@ -1281,7 +1386,6 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
   uint64_t* far_out = c.far[far_sel];
   uint32_t* far_out_cnt = c.far_cnt + far_sel;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
-  const int32_t next = *c.iter + 1;
   const int tid = threadIdx.x;
   const int32_t CH = 1 << c.chunk_shift;
   // nLb: one-label terms of the completion bounds of improved states
@@ -1325,18 +1429,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
       else if (HOP && c.hop_prune && best < INF && t + tl.y > best) expand = false;
       else if (!HOP) {
         const bool half = t < best / 2 + 2;  // 1/2 rule: costlier states are never expanded
-        if (ch == 0 && (!half || f < sc->thr)) {
-          const uint32_t xs = (uint32_t)S.full ^ p;
-          const int32_t t2 = c.T[2 * t_ix(S, xs, v)];
-          if (t2 < INF && t < INF) {
-            const int32_t mcost = t + t2;
-            int32_t* dfull = &c.T[2 * t_ix(S, S.full, v)];
-            if (mcost < *dfull && mcost < atomicMin(dfull, mcost)) {
-              if (c.BP) c.BP[e_ix(S, S.full, v)] = (int32_t)bp_merge(p, 0, 0);
-              ub_update(&c.bestw[s], mcost, wit_full(v, 0));
-            }
-          }
-        }
+        if (ch == 0 && (!half || f < sc->thr)) complement_merge(c, S, s, v, p, t);
         if (!half) expand = false;
       }
       if (expand && f >= sc->thr) {  // not in the current band yet
@@ -1349,9 +1442,9 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
           }
         } else {
           if (f < sc->mindef) atomicMin(&sc->mindef, f);
-          if (atomicMax(&c.E[e_ix(S, row, v)], next) < next) {
-            carry_cnt = nchunks(c, v);
-            citem = mk_item(v, p, d, 0, s);
+          if (ch == 0 && mark_state(c, S, v, p, d)) {
+            carry_cnt = 1;
+            citem = mk_item(v, 0, d, 0, s);
           }
         }
       }
@@ -1484,9 +1577,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
             if (HOP && p2 == mk) {  // a full tree
               if (mc < atomicMin(dst, mc)) {
                 ++nU;
-                if (c.BP)
-                  c.BP[c.slots[sl].e_off + (s_tb[j] - c.slots[sl].t_off) + drow] =
-                      (int32_t)bp_merge(pj, dj, d2q[q]);
+                if (c.BP) c.BP[s_tb[j] + drow] = (int32_t)bp_merge(pj, dj, d2q[q]);
                 ub_update(&c.bestw[sl], mc, wit_full(vj, nd));
               }
             } else {
@@ -1503,7 +1594,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                 ++nU;
                 nLb += (uint32_t)__popc(mk ^ p2);  // the improved state's bound terms
                 const SlotDesc Sj = c.slots[sl];
-                if (c.BP) c.BP[e_ix(Sj, drow, vj)] = (int32_t)bp_merge(pj, dj, d2q[q]);
+                if (c.BP) c.BP[s_tb[j] + drow] = (int32_t)bp_merge(pj, dj, d2q[q]);
                 if (!HOP) {
                   ++nUb;
                   const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)vj * CROW], mk ^ p2,
@@ -1513,15 +1604,17 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                     best = mc + sumc;
                   }
                 }
-                // 2/3 rule (Conditional Tree Merging) for the queue only
+                // 2/3 rule (Conditional Tree Merging) for the queue only; a state
+                // left out of the queue is combined with its complement now
                 const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
-                if (enq && atomicMax(&c.E[e_ix(Sj, drow, vj)], next) < next) {
+                if (!enq) complement_merge(c, Sj, sl, vj, P, mc);
+                if (enq) {
                   if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[sl].thr) {
                     fwant = 1;  // past the band: wait in the far list
                     fitem = mk_item(vj, P, nd, 0, sl);
-                  } else {
-                    want = nchunks(c, vj);
-                    item = mk_item(vj, P, nd, 0, sl);
+                  } else if (mark_state(c, Sj, vj, P, nd)) {
+                    want = 1;
+                    item = mk_item(vj, 0, nd, 0, sl);
                   }
                 }
               }
@@ -1601,7 +1694,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
               ++nU;
               nLb += meta >> 24;  // the improved state's bound terms
               const SlotDesc Sj = c.slots[sl];
-              if (c.BP) c.BP[dbase - Sj.t_off + Sj.e_off + uoff] = (int32_t)bp_grow(s_v[j]);
+              if (c.BP) c.BP[dbase + uoff] = (int32_t)bp_grow(s_v[j]);
               if (!HOP) {
                 ++nUb;
                 const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)u * CROW],
@@ -1609,15 +1702,12 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
                 if (sumc < INF && gc + sumc < best)
                   ub_update(&c.bestw[sl], gc + sumc, wit_onelabel(u, pd));
               }
-              const int64_t erow = dbase - Sj.t_off + Sj.e_off;
-              if (atomicMax(&c.E[erow + uoff], next) < next) {
-                if (far_mode && min(gc + dl.y, INF - 1) >= c.sched[sl].thr) {
-                  fwant = 1;  // past the band: wait in the far list
-                  fitem = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
-                } else {
-                  want = nchunks(c, u);
-                  item = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
-                }
+              if (far_mode && min(gc + dl.y, INF - 1) >= c.sched[sl].thr) {
+                fwant = 1;  // past the band: wait in the far list
+                fitem = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+              } else if (mark_state(c, Sj, u, pd, HOP ? dd + 1 : 0)) {
+                want = 1;
+                item = mk_item(u, 0, HOP ? dd + 1 : 0, 0, sl);
               }
             }
           }
@@ -1780,7 +1870,7 @@ __global__ __launch_bounds__(64) void k_backtrack(DevCtx c, int max_edges, int32
     if ((p & ~gs) == 0) continue;  // p inside v's groups: a single vertex of cost 0
     if (c.BP) {
       // the stored backpointer, used when it checks out against the final table
-      const uint32_t bp = (uint32_t)c.BP[e_ix(S, (int64_t)d * S.nsub + p, v)];
+      const uint32_t bp = (uint32_t)c.BP[t_ix(S, (int64_t)d * S.nsub + p, v)];
       bool used = false;
       if (bp != BP_BASE && (bp >> 31)) {
         const uint32_t p1 = bp & 0x7Fu, d1 = (bp >> 7) & 0xFu, d2 = (bp >> 11) & 0xFu;
