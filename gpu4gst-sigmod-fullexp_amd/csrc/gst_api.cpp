@@ -284,6 +284,10 @@ DevCtx make_ctx(gst_graph* g) {
 
 int expand_grid(const gst_graph* g) { return g->grid_eff > 0 ? g->grid_eff : g->cus * 8; }
 
+// The unfold step is light per item (a mask read and a few stores): two
+// workgroups per CU keep its launch cheap when the frontier is small.
+int unfold_grid(const gst_graph* g) { return std::min(expand_grid(g), g->cus * 2); }
+
 // Size-class model of the expand settings, from sweeps on the 168k-vertex /
 // 6.8M-edge power-law workload (chunk 128..8192, grid 2..16 x CUs): a lone
 // query has little parallelism besides its hub rows, so they are split into
@@ -337,7 +341,8 @@ void launch_expand_step(gst_graph* g, const DevCtx& c, int mode, int h) {
     launch_sssp_expand(c, WL{l[h], cnt + h, l[1 - h], cnt + (1 - h)}, grid, g->stream);
     return;
   }
-  launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, grid, g->stream);
+  launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, unfold_grid(g),
+                   g->stream);
   launch_dp_expand(c, WL{g->wls.as<uint64_t>(), cnt + kCntStates, l[1 - h], cnt + (1 - h)}, grid,
                    mode == 2, g->stream);
 }
@@ -371,7 +376,7 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
       for (int h = 0; h < 2; ++h) {
         // the DP's unfold step runs before the timed expand launch
         if (mode != 0)
-          launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, grid,
+          launch_dp_unfold(c, l[h], cnt + h, g->wls.as<uint64_t>(), cnt + kCntStates, unfold_grid(g),
                            g->stream);
         uint32_t items = 0;
         if (g->trace) {

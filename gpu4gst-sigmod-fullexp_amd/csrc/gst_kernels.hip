@@ -790,42 +790,61 @@ __global__ __launch_bounds__(BLOCK) void k_dp_unfold(DevCtx c, const uint64_t* v
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ uint32_t s_base;
+  __shared__ uint4 s_w[BLOCK];       // the item's pending mask (the queued flag cleared)
+  __shared__ uint32_t s_pc[BLOCK];   // its number of pending states
+  __shared__ uint64_t s_it[BLOCK];   // the vertex item (v, d, slot)
   const uint32_t n = min(*vin_cnt, c.cap);
   const int tid = threadIdx.x;
   for (uint32_t tile = blockIdx.x; tile * BLOCK < n; tile += gridDim.x) {
     const uint32_t idx = tile * BLOCK + tid;
-    uint32_t w[4] = {0u, 0u, 0u, 0u};
-    uint32_t v = 0, d = 0, s = 0, nch = 0, mw = 0;
     int32_t cnt = 0;
     if (idx < n) {
       const uint64_t it = vin[idx];
-      v = it_v(it);
-      d = it_d(it);
-      s = it_s(it);
-      const SlotDesc S = c.slots[s];
-      mw = (uint32_t)S.mw;
+      const uint32_t v = it_v(it), d = it_d(it);
+      const SlotDesc S = c.slots[it_s(it)];
       uint32_t* m = c.M + S.m_off + ((int64_t)d * c.Vp + v) * S.mw;
-      for (uint32_t i = 0; i < mw; ++i) {
+      uint32_t w[4] = {0u, 0u, 0u, 0u};
+      for (int i = 0; i < S.mw; ++i) {
         w[i] = m[i];
         m[i] = 0u;
       }
       w[0] &= ~1u;  // the queued flag is no state
-      const int32_t pc = __popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]);
-      nch = nchunks(c, v);
-      cnt = pc * (int32_t)nch;
+      const uint32_t pc = (uint32_t)(__popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]));
+      s_w[tid] = make_uint4(w[0], w[1], w[2], w[3]);
+      s_pc[tid] = pc;
+      s_it[tid] = it;
+      cnt = (int32_t)(pc * nchunks(c, v));
     }
     block_scan(cnt, s_off, s_wsum);
-    if (tid == 0) s_base = s_off[BLOCK] ? atomicAdd(sout_cnt, (uint32_t)s_off[BLOCK]) : 0u;
+    const uint32_t total = (uint32_t)s_off[BLOCK];
+    if (tid == 0) s_base = total ? atomicAdd(sout_cnt, total) : 0u;
     __syncthreads();
-    uint32_t pos = s_base + (uint32_t)s_off[tid];
-    for (uint32_t ch = 0; ch < nch; ++ch)
-      for (uint32_t i = 0; i < mw; ++i)
-        for (uint32_t rest = w[i]; rest; rest &= rest - 1u, ++pos) {
-          const uint32_t p = 32u * i + (uint32_t)__builtin_ctz(rest);
-          if (pos < c.cap) sout[pos] = mk_item(v, p, d, ch, s);
-          else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+    // output position e of the tile: item j, chunk r / pc, the (r % pc)-th
+    // pending state; consecutive lanes write consecutive positions
+    for (uint32_t e = (uint32_t)tid; e < total; e += BLOCK) {
+      const int j = find_item(s_off, (int32_t)e);
+      const uint32_t r = e - (uint32_t)s_off[j], pc = s_pc[j];
+      const uint32_t ch = r / pc;
+      uint32_t k = r - ch * pc;
+      const uint4 w4 = s_w[j];
+      const uint32_t ws[4] = {w4.x, w4.y, w4.z, w4.w};
+      uint32_t p = 0;
+      for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t pcw = (uint32_t)__popc(ws[i]);
+        if (k < pcw) {
+          uint32_t x = ws[i];
+          for (; k; --k) x &= x - 1u;  // drop the k lowest set bits
+          p = 32u * i + (uint32_t)__builtin_ctz(x);
+          break;
         }
-    __syncthreads();  // s_off / s_base are reused by the next tile
+        k -= pcw;
+      }
+      const uint64_t it = s_it[j];
+      const uint32_t pos = s_base + e;
+      if (pos < c.cap) sout[pos] = mk_item(it_v(it), p, it_d(it), ch, it_s(it));
+      else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+    }
+    __syncthreads();  // the LDS arrays are reused by the next tile
   }
 }
 
