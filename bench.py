@@ -151,6 +151,8 @@ def _apply_options(G, args):
         G.set_option("lane_appends", 1)
     if args.per_item_edges:
         G.set_option("per_item_edges", 1)
+    if args.rolling >= 0:
+        G.set_option("rolling", args.rolling)
 
 
 def _write_trace(G, path):
@@ -197,7 +199,7 @@ def _dp_stats(st):
     return {k: st[k] for k in ("sssp_iterations", "sssp_items", "sssp_relax", "sssp_updates",
                                "sssp_items_unbounded", "dp_iterations", "states_expanded",
                                "grow_relax", "merge_relax", "improvements", "max_frontier",
-                               "batches")}
+                               "batches", "reruns", "rolling_queries", "max_resident")}
 
 
 def other_workload(args):
@@ -318,6 +320,9 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a device copy of this size (counter calibration) and exit")
     ap.add_argument("--single-dp-div", type=int, default=-1)
+    ap.add_argument("--rolling", type=int, default=-1,
+                    help="1 = rolling admission when a call's tables do not fit at once (the "
+                         "library default), 0 = lockstep admission batches")
     ap.add_argument("--trace", default="",
                     help="write the per-launch trace of the profiled pass to this CSV file")
     ap.add_argument("--workload", type=int, default=2,

@@ -6,6 +6,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <map>
 #include <string>
 #include <utility>
@@ -159,6 +160,15 @@ struct gst_graph {
   DBuf out_uv, out_eid, out_n, out_err;
   uint32_t* h_cnt = nullptr;  // pinned
   int32_t h_nslots[1] = {0};  // slots of the batch being solved
+  // Rolling admission (GST_OPT_ROLLING): a call whose tables do not fit the
+  // HBM budget at once keeps a pool of DP table regions busy, admitting the
+  // next query as soon as one converges. `resident` = queries holding a
+  // region now (0 outside a rolling run).
+  int32_t rolling = 1;
+  int32_t resident = 0;
+  int32_t out_rows = 0;  // rows of the tree-recovery output buffers
+  DBuf rsel, rtasks;  // slot lists / tasks of the queries admitted or collected
+  DBuf rbest;         // upper bounds after the one-label phase
   uint32_t wl_cap = 0;
   int64_t gen = 0;  // bumps whenever a pointer or scalar baked into a captured graph changes
   std::map<std::pair<int, int>, std::pair<int64_t, hipGraphExec_t>> graphs;
@@ -361,15 +371,13 @@ void launch_pair(gst_graph* g, const DevCtx& c, int mode) {
   }
 }
 
-// Runs frontier iterations until the worklist is empty. mode 0 = one-label,
-// 1 = plain DP, 2 = hop DP. Kernel timings are accumulated in profiling mode.
-int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
-  DevCtx c = make_ctx(g);
-  int pairs = 1;
-  const int max_pairs = std::max(1, g->iters_per_sync / 2);
-  int64_t total_pairs = 0;
+// One round of `pairs` frontier iteration pairs: replayed from a captured
+// graph, launched one by one, or (profiling mode) launched one by one with
+// every expand launch timed (one pair; `pairs` is set to the pairs run).
+int phase_round(gst_graph* g, const DevCtx& c, int mode, int& pairs, double* ms_acc,
+                int64_t* launches) {
   uint32_t* cnt = g->cnt.as<uint32_t>();
-  while (true) {
+  {
     if (g->profile) {
       const int grid = expand_grid(g);
       uint64_t* l[2] = {g->wl0.as<uint64_t>(), g->wl1.as<uint64_t>()};
@@ -407,7 +415,7 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
           r.batch = g->batch_no;
           r.items = items;
           r.active_slots = (int32_t)active;
-          r.slots = *g->h_nslots;
+          r.slots = g->resident > 0 ? g->resident : *g->h_nslots;
           r.ms = ms;
           g->trace_rows.push_back(r);
         }
@@ -437,6 +445,21 @@ int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
       for (int r = 0; r < pairs; ++r) launch_pair(g, c, mode);
     }
     HIP_TRY(hipGetLastError());
+  }
+  return GST_OK;
+}
+
+// Runs frontier iterations until the worklist is empty. mode 0 = one-label,
+// 1 = plain DP, 2 = hop DP. Kernel timings are accumulated in profiling mode.
+int run_phase(gst_graph* g, int mode, double* ms_acc, int64_t* launches) {
+  DevCtx c = make_ctx(g);
+  int pairs = 1;
+  const int max_pairs = std::max(1, g->iters_per_sync / 2);
+  int64_t total_pairs = 0;
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  while (true) {
+    const int prc = phase_round(g, c, mode, pairs, ms_acc, launches);
+    if (prc) return prc;
     total_pairs += pairs;
     HIP_TRY(hipMemcpyAsync(g->h_cnt, cnt, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
     HIP_TRY(hipMemcpyAsync(g->h_cnt + 2, g->ctr.as<uint64_t>() + C_OVERFLOW, sizeof(uint64_t),
@@ -479,6 +502,194 @@ struct BatchOut {
   std::vector<std::vector<std::pair<int32_t, int32_t>>> uv;
   std::vector<std::vector<int32_t>> ids;
 };
+
+// Tree recovery and results of the slots `sel` (host list; `d_sel` the same
+// list on the device, or null when sel = 0, 1, ..., n-1): output row i of the
+// recovery kernel belongs to slot sel[i]. Costs, statuses and trees go to the
+// queries' entries of `out`.
+int collect_chunk(gst_graph* g, const DevCtx& c, const std::vector<QueryIn>& qs,
+                  const std::vector<int32_t>& sel, const int32_t* d_sel, BatchOut& out) {
+  const int n = (int)sel.size();
+  if (n == 0) return GST_OK;
+  const int nslots = (int)qs.size();
+  const int maxe = g->max_tree_edges;
+  launch_backtrack(c, n, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
+                   g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), d_sel, g->stream);
+  HIP_TRY(hipGetLastError());
+  std::vector<uint64_t> bw(nslots);
+  std::vector<int32_t> on(n), oerr(n);
+  std::vector<Sched> hsched(nslots);
+  HIP_TRY(hipMemcpyAsync(bw.data(), g->bestw.p, nslots * 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(on.data(), g->out_n.p, n * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(oerr.data(), g->out_err.p, n * 4, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipMemcpyAsync(hsched.data(), g->sched.p, nslots * sizeof(Sched), hipMemcpyDeviceToHost,
+                         g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  // only the filled prefix of every row comes back
+  int32_t max_on = 0;
+  for (int o = 0; o < n; ++o)
+    if (oerr[o] == 0) max_on = std::max(max_on, std::min(on[o], maxe));
+  std::vector<int32_t> huv((size_t)n * std::max(max_on, 1) * 2),
+      hid((size_t)n * std::max(max_on, 1));
+  if (max_on > 0) {
+    HIP_TRY(hipMemcpy2D(huv.data(), (size_t)max_on * 8, g->out_uv.p, (size_t)maxe * 8,
+                        (size_t)max_on * 8, n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy2D(hid.data(), (size_t)max_on * 4, g->out_eid.p, (size_t)maxe * 4,
+                        (size_t)max_on * 4, n, hipMemcpyDeviceToHost));
+  }
+  // Trees longer than the per-slot buffer (code 2) are recovered again with
+  // room for min(cost, 4V) edges: every recovered edge weighs >= 1 and their
+  // total is at most the cost.
+  std::vector<int32_t> redo;  // slots
+  for (int o = 0; o < n && !g->noalloc; ++o)
+    if (oerr[o] == 2) redo.push_back(sel[o]);
+  std::map<int, std::pair<std::vector<int32_t>, std::vector<int32_t>>> long_trees;  // by slot
+  std::map<int, int32_t> redo_err;  // by slot
+  if (!redo.empty()) {
+    int64_t cap2 = 1;
+    for (int s : redo)
+      cap2 = std::max<int64_t>(cap2, std::min<int64_t>((int64_t)(bw[s] >> 32) + 1,
+                                                        4 * (int64_t)g->V + 64));
+    const int nr = (int)redo.size();
+    DBuf uv2, id2, n2, e2, rs;
+    auto release = [&]() {
+      for (DBuf* b : {&uv2, &id2, &n2, &e2, &rs}) b->release();
+    };
+    hipError_t he = uv2.ensure((size_t)nr * cap2 * 8);
+    if (he == hipSuccess) he = id2.ensure((size_t)nr * cap2 * 4);
+    if (he == hipSuccess) he = n2.ensure((size_t)nr * 4);
+    if (he == hipSuccess) he = e2.ensure((size_t)nr * 4);
+    if (he == hipSuccess) he = rs.ensure((size_t)nr * 4);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(rs.p, redo.data(), nr * 4, hipMemcpyHostToDevice, g->stream);
+    if (he == hipSuccess) {
+      launch_backtrack(c, nr, (int)cap2, uv2.as<int32_t>(), id2.as<int32_t>(), n2.as<int32_t>(),
+                       e2.as<int32_t>(), rs.as<int32_t>(), g->stream);
+      he = hipGetLastError();
+    }
+    std::vector<int32_t> n2h(nr), e2h(nr), uvh, idh;
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(n2h.data(), n2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(e2h.data(), e2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(g->stream);
+    for (int r = 0; r < nr && he == hipSuccess; ++r) {
+      const int s = redo[r];
+      redo_err[s] = e2h[r];
+      if (e2h[r] != 0) continue;
+      uvh.resize((size_t)n2h[r] * 2);
+      idh.resize((size_t)n2h[r]);
+      he = hipMemcpy(uvh.data(), uv2.as<int32_t>() + (size_t)r * cap2 * 2, uvh.size() * 4,
+                     hipMemcpyDeviceToHost);
+      if (he == hipSuccess)
+        he = hipMemcpy(idh.data(), id2.as<int32_t>() + (size_t)r * cap2, idh.size() * 4,
+                       hipMemcpyDeviceToHost);
+      long_trees[s] = {uvh, idh};
+    }
+    release();
+    HIP_TRY(he);
+  }
+  for (int o = 0; o < n; ++o) {
+    const int s = sel[o];
+    const int qi = qs[s].idx;
+    const int32_t cost = (int32_t)(bw[s] >> 32);
+    int32_t err = oerr[o];
+    if (err == 2 && redo_err.count(s)) err = redo_err[s];
+    if (err == 2) {  // still longer than the retry's buffer: cost only
+      out.cost[qi] = cost;
+      out.status[qi] = GST_Q_NO_TREE;
+      out.uv[qi].clear();
+      out.ids[qi].clear();
+      continue;
+    }
+    if (err != 0)
+      return fail(GST_ERR_INTERNAL, "tree recovery failed for query " + std::to_string(qi) +
+                                        " (code " + std::to_string(err) + ")");
+    if (cost >= INF) {
+      out.cost[qi] = g->infeasible_cost;
+      out.status[qi] = hsched[s].sat ? GST_Q_RANGE : GST_Q_INFEASIBLE;
+      continue;
+    }
+    out.cost[qi] = cost;
+    out.status[qi] = GST_Q_OK;
+    std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> es;
+    auto lt = long_trees.find(s);
+    if (lt != long_trees.end()) {
+      const auto& uvs = lt->second.first;
+      const auto& ids = lt->second.second;
+      for (size_t i = 0; i < ids.size(); ++i)
+        es.push_back({{std::min(uvs[2 * i], uvs[2 * i + 1]), std::max(uvs[2 * i], uvs[2 * i + 1])},
+                      ids[i]});
+    }
+    for (int i = 0; lt == long_trees.end() && i < on[o]; ++i) {
+      int32_t a = huv[((size_t)o * max_on + i) * 2], b = huv[((size_t)o * max_on + i) * 2 + 1];
+      es.push_back({{std::min(a, b), std::max(a, b)}, hid[(size_t)o * max_on + i]});
+    }
+    std::sort(es.begin(), es.end());
+    es.erase(std::unique(es.begin(), es.end(),
+                         [](const auto& x, const auto& y) { return x.first == y.first; }),
+             es.end());
+    out.uv[qi].clear();
+    out.ids[qi].clear();
+    for (auto& e : es) {
+      out.uv[qi].push_back(e.first);
+      out.ids[qi].push_back(e.second);
+    }
+  }
+  return GST_OK;
+}
+
+// The one-label phase of slots 0..nslots-1 (tasks in slot order), in passes
+// of per_pass slots, then the clamped distances, group distances and route
+// tables of every slot. Counters and list counts start from zero; ph[1] is
+// recorded when the last pass ends.
+int onelabel_stage(gst_graph* g, const DevCtx& c, const std::vector<Task>& tasks, int nslots,
+                   int per_pass, double* sssp_ms, int64_t* sssp_launches) {
+  launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
+  launch_fill_i32(g->cnt.as<int32_t>(), kCntWords, 0, g->stream);
+  launch_init_slot_tables(c, nslots, (int64_t)nslots * g->Vp * CROW, g->stream);
+  const Task* dt = g->tasks.as<Task>();
+  const int nt = (int)tasks.size();
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  // The one-label phase runs over groups of per_pass slots at a time (tasks
+  // are in slot order): the label rows one pass touches can stay in the
+  // last-level cache, and the lists need room for one pass only.
+  launch_fill_i32(g->cnt.as<int32_t>() + 2, 2, 0, g->stream);   // far list counts
+  launch_fill_i32(g->iter.as<int32_t>() + 2, 1, 0, g->stream);  // far list in use
+  launch_band_start(c, g->stream);
+  for (int t0 = 0; t0 < nt;) {
+    const int s_end = std::min(nslots, tasks[t0].slot + per_pass);
+    int t1 = t0;
+    while (t1 < nt && tasks[t1].slot < s_end) ++t1;
+    launch_groups_setup(c, dt + t0, t1 - t0, g->goff.as<int64_t>(), g->gmem.as<int32_t>(),
+                        g->wl0.as<uint64_t>(), cnt, g->stream);
+    HIP_TRY(hipGetLastError());
+    int rc = run_phase(g, 0, sssp_ms, sssp_launches);
+    if (rc) return rc;  // includes 100 = overflow
+    t0 = t1;
+  }
+  HIP_TRY(hipEventRecord(g->ph[1], g->stream));
+  launch_clamp(c, nslots, g->stream);
+  launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
+  launch_route(c, nslots, g->stream);
+  HIP_TRY(hipGetLastError());
+  return GST_OK;
+}
+
+// collect_chunk over pieces of `sel` that fit the output rows.
+int collect_results(gst_graph* g, const DevCtx& c, const std::vector<QueryIn>& qs,
+                    const std::vector<int32_t>& sel, const int32_t* d_sel, BatchOut& out) {
+  const size_t rows = (size_t)std::max(1, g->out_rows);
+  if ((int64_t)sel.size() <= (int64_t)rows) return collect_chunk(g, c, qs, sel, d_sel, out);
+  if (!d_sel) return fail(GST_ERR_INTERNAL, "tree recovery rows too few for the batch");
+  for (size_t a = 0; a < sel.size(); a += rows) {
+    const size_t b = std::min(sel.size(), a + rows);
+    const std::vector<int32_t> part(sel.begin() + a, sel.begin() + b);
+    const int rc = collect_chunk(g, c, qs, part, d_sel + a, out);
+    if (rc) return rc;
+  }
+  return GST_OK;
+}
 
 int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac, BatchOut& out,
                    gst_stats* st) {
@@ -579,6 +790,7 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   HIP_TRY(g->out_eid.ensure((size_t)nslots * maxe * 4, &grew));
   HIP_TRY(g->out_n.ensure((size_t)nslots * 4, &grew));
   HIP_TRY(g->out_err.ensure((size_t)nslots * 4, &grew));
+  g->out_rows = nslots;
   if (grew) {
     ++g->gen;
     if (!g->noalloc) g->drop_graphs();
@@ -606,36 +818,16 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   }
   ++g->batch_no;
   HIP_TRY(hipEventRecord(g->ph[0], g->stream));
-  launch_fill_u64(g->ctr.as<uint64_t>(), C_NUM, 0, g->stream);
-  launch_fill_i32(g->cnt.as<int32_t>(), kCntWords, 0, g->stream);
-  launch_init_slot_tables(c, nslots, (int64_t)nslots * g->Vp * CROW, g->stream);
   const Task* dt = g->tasks.as<Task>();
   const int nt = (int)tasks.size();
   uint32_t* cnt = g->cnt.as<uint32_t>();
   double sssp_ms = 0, dp_ms = 0;
   int64_t dp_launches = 0, sssp_launches = 0;
-  // The one-label phase runs over groups of sssp_slots slots at a time (tasks
-  // are in slot order), so that the label rows one pass touches can stay in
-  // the last-level cache; 0 = all slots in one pass.
-  launch_fill_i32(g->cnt.as<int32_t>() + 2, 2, 0, g->stream);   // far list counts
-  launch_fill_i32(g->iter.as<int32_t>() + 2, 1, 0, g->stream);  // far list in use
-  launch_band_start(c, g->stream);
-  const int per_pass = g->sssp_slots > 0 ? g->sssp_slots : nslots;
-  for (int t0 = 0; t0 < nt;) {
-    const int s_end = std::min(nslots, tasks[t0].slot + per_pass);
-    int t1 = t0;
-    while (t1 < nt && tasks[t1].slot < s_end) ++t1;
-    launch_groups_setup(c, dt + t0, t1 - t0, g->goff.as<int64_t>(), g->gmem.as<int32_t>(),
-                        g->wl0.as<uint64_t>(), cnt, g->stream);
-    HIP_TRY(hipGetLastError());
-    int rc = run_phase(g, 0, &sssp_ms, &sssp_launches);
+  {
+    const int rc = onelabel_stage(g, c, tasks, nslots, g->sssp_slots > 0 ? g->sssp_slots : nslots,
+                                  &sssp_ms, &sssp_launches);
     if (rc) return rc;  // includes 100 = overflow
-    t0 = t1;
   }
-  HIP_TRY(hipEventRecord(g->ph[1], g->stream));
-  launch_clamp(c, nslots, g->stream);
-  launch_vv(c, dt, nt, g->goff.as<int64_t>(), g->gmem.as<int32_t>(), g->stream);
-  launch_route(c, nslots, g->stream);
   launch_lb_table(c, nslots, g->stream);
   // pending masks start empty (an overflowed earlier run may have left bits)
   launch_fill_i32(reinterpret_cast<int32_t*>(g->Ms.p), m_tot, 0, g->stream);
@@ -649,131 +841,487 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
   int rc = run_phase(g, qs[0].D >= 0 ? 2 : 1, &dp_ms, &dp_launches);
   if (rc) return rc;
   HIP_TRY(hipEventRecord(g->ph[3], g->stream));
-  launch_backtrack(c, nslots, maxe, g->out_uv.as<int32_t>(), g->out_eid.as<int32_t>(),
-                   g->out_n.as<int32_t>(), g->out_err.as<int32_t>(), nullptr, g->stream);
-  HIP_TRY(hipGetLastError());
-  std::vector<uint64_t> ctr(C_NUM), bw(nslots);
-  std::vector<int32_t> on(nslots), oerr(nslots);
-  std::vector<Sched> hsched(nslots);
+  std::vector<uint64_t> ctr(C_NUM);
   HIP_TRY(hipMemcpyAsync(ctr.data(), g->ctr.p, C_NUM * 8, hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipMemcpyAsync(bw.data(), g->bestw.p, nslots * 8, hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipMemcpyAsync(on.data(), g->out_n.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipMemcpyAsync(oerr.data(), g->out_err.p, nslots * 4, hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipMemcpyAsync(hsched.data(), g->sched.p, nslots * sizeof(Sched), hipMemcpyDeviceToHost,
-                         g->stream));
   HIP_TRY(hipStreamSynchronize(g->stream));
   if (ctr[C_OVERFLOW]) return 100;  // worklist too small: caller retries with full capacity
+  {
+    std::vector<int32_t> all(nslots);
+    for (int s = 0; s < nslots; ++s) all[s] = s;
+    int crc = collect_results(g, c, qs, all, nullptr, out);
+    if (crc) return crc;
+  }
+  // again: the tree recovery counts its backpointer misses
+  HIP_TRY(hipMemcpy(ctr.data(), g->ctr.p, C_NUM * 8, hipMemcpyDeviceToHost));
   float ph_ms[3] = {0, 0, 0};
   for (int i = 0; i < 3; ++i) HIP_TRY(hipEventElapsedTime(&ph_ms[i], g->ph[i], g->ph[i + 1]));
-  // only the filled prefix of every slot's edge row comes back
-  int32_t max_on = 0;
-  for (int s = 0; s < nslots; ++s)
-    if (oerr[s] == 0) max_on = std::max(max_on, std::min(on[s], maxe));
-  std::vector<int32_t> huv((size_t)nslots * std::max(max_on, 1) * 2),
-      hid((size_t)nslots * std::max(max_on, 1));
-  if (max_on > 0) {
-    HIP_TRY(hipMemcpy2D(huv.data(), (size_t)max_on * 8, g->out_uv.p, (size_t)maxe * 8,
-                        (size_t)max_on * 8, nslots, hipMemcpyDeviceToHost));
-    HIP_TRY(hipMemcpy2D(hid.data(), (size_t)max_on * 4, g->out_eid.p, (size_t)maxe * 4,
-                        (size_t)max_on * 4, nslots, hipMemcpyDeviceToHost));
-  }
-  // Trees longer than the per-slot buffer (code 2) are recovered again with
-  // room for min(cost, 4V) edges: every recovered edge weighs >= 1 and their
-  // total is at most the cost.
-  std::vector<int32_t> redo;
-  for (int s = 0; s < nslots && !g->noalloc; ++s)
-    if (oerr[s] == 2) redo.push_back(s);
-  std::map<int, std::pair<std::vector<int32_t>, std::vector<int32_t>>> long_trees;
-  if (!redo.empty()) {
-    int64_t cap2 = 1;
-    for (int s : redo)
-      cap2 = std::max<int64_t>(cap2, std::min<int64_t>((int64_t)(bw[s] >> 32) + 1,
-                                                        4 * (int64_t)g->V + 64));
-    const int nr = (int)redo.size();
-    DBuf uv2, id2, n2, e2, sel;
-    auto release = [&]() {
-      for (DBuf* b : {&uv2, &id2, &n2, &e2, &sel}) b->release();
-    };
-    hipError_t he = uv2.ensure((size_t)nr * cap2 * 8);
-    if (he == hipSuccess) he = id2.ensure((size_t)nr * cap2 * 4);
-    if (he == hipSuccess) he = n2.ensure((size_t)nr * 4);
-    if (he == hipSuccess) he = e2.ensure((size_t)nr * 4);
-    if (he == hipSuccess) he = sel.ensure((size_t)nr * 4);
-    if (he == hipSuccess)
-      he = hipMemcpyAsync(sel.p, redo.data(), nr * 4, hipMemcpyHostToDevice, g->stream);
-    if (he == hipSuccess) {
-      launch_backtrack(c, nr, (int)cap2, uv2.as<int32_t>(), id2.as<int32_t>(), n2.as<int32_t>(),
-                       e2.as<int32_t>(), sel.as<int32_t>(), g->stream);
-      he = hipGetLastError();
-    }
-    std::vector<int32_t> n2h(nr), e2h(nr), uvh, idh;
-    if (he == hipSuccess)
-      he = hipMemcpyAsync(n2h.data(), n2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
-    if (he == hipSuccess)
-      he = hipMemcpyAsync(e2h.data(), e2.p, nr * 4, hipMemcpyDeviceToHost, g->stream);
-    if (he == hipSuccess) he = hipStreamSynchronize(g->stream);
-    for (int r = 0; r < nr && he == hipSuccess; ++r) {
-      const int s = redo[r];
-      oerr[s] = e2h[r];
-      if (e2h[r] != 0) continue;
-      uvh.resize((size_t)n2h[r] * 2);
-      idh.resize((size_t)n2h[r]);
-      he = hipMemcpy(uvh.data(), uv2.as<int32_t>() + (size_t)r * cap2 * 2, uvh.size() * 4,
-                     hipMemcpyDeviceToHost);
-      if (he == hipSuccess)
-        he = hipMemcpy(idh.data(), id2.as<int32_t>() + (size_t)r * cap2, idh.size() * 4,
-                       hipMemcpyDeviceToHost);
-      long_trees[s] = {uvh, idh};
-    }
-    release();
-    HIP_TRY(he);
-  }
-  for (int s = 0; s < nslots; ++s) {
-    const int qi = qs[s].idx;
-    const int32_t cost = (int32_t)(bw[s] >> 32);
-    if (oerr[s] == 2) {  // still longer than the retry's buffer: cost only
-      out.cost[qi] = cost;
-      out.status[qi] = GST_Q_NO_TREE;
-      out.uv[qi].clear();
-      out.ids[qi].clear();
-      continue;
-    }
-    if (oerr[s] != 0)
-      return fail(GST_ERR_INTERNAL, "tree recovery failed for query " + std::to_string(qi) +
-                                        " (code " + std::to_string(oerr[s]) + ")");
-    if (cost >= INF) {
-      out.cost[qi] = g->infeasible_cost;
-      out.status[qi] = hsched[s].sat ? GST_Q_RANGE : GST_Q_INFEASIBLE;
-      continue;
-    }
-    out.cost[qi] = cost;
-    out.status[qi] = GST_Q_OK;
-    std::vector<std::pair<std::pair<int32_t, int32_t>, int32_t>> es;
-    auto lt = long_trees.find(s);
-    if (lt != long_trees.end()) {
-      const auto& uvs = lt->second.first;
-      const auto& ids = lt->second.second;
-      for (size_t i = 0; i < ids.size(); ++i)
-        es.push_back({{std::min(uvs[2 * i], uvs[2 * i + 1]), std::max(uvs[2 * i], uvs[2 * i + 1])},
-                      ids[i]});
-    }
-    for (int i = 0; lt == long_trees.end() && i < on[s]; ++i) {
-      int32_t a = huv[((size_t)s * max_on + i) * 2], b = huv[((size_t)s * max_on + i) * 2 + 1];
-      es.push_back({{std::min(a, b), std::max(a, b)}, hid[(size_t)s * max_on + i]});
-    }
-    std::sort(es.begin(), es.end());
-    es.erase(std::unique(es.begin(), es.end(),
-                         [](const auto& x, const auto& y) { return x.first == y.first; }),
-             es.end());
-    out.uv[qi].clear();
-    out.ids[qi].clear();
-    for (auto& e : es) {
-      out.uv[qi].push_back(e.first);
-      out.ids[qi].push_back(e.second);
-    }
-  }
   if (st) {
     st->batches += 1;
+    st->queries += nslots;
+    st->sssp_iterations += (int64_t)ctr[C_SSSP_ITERS];
+    st->dp_iterations += (int64_t)ctr[C_DP_ITERS];
+    st->sssp_items += (int64_t)ctr[C_SSSP_ITEMS];
+    st->sssp_relax += (int64_t)ctr[C_SSSP_RELAX];
+    st->sssp_updates += (int64_t)ctr[C_SSSP_UPD];
+    st->sssp_items_unbounded += (int64_t)ctr[C_SSSP_NOUB];
+    st->states_expanded += (int64_t)ctr[C_F];
+    st->grow_relax += (int64_t)ctr[C_RG];
+    st->merge_relax += (int64_t)ctr[C_RM];
+    st->improvements += (int64_t)ctr[C_U];
+    st->lb_terms += (int64_t)ctr[C_LB];
+    st->ub_rows += (int64_t)ctr[C_UBROW];
+    st->bp_fallbacks += (int64_t)ctr[C_BP_MISS];
+    st->backpointers_stored = g->bp_call ? 1 : 0;
+    st->items_processed += (int64_t)ctr[C_ITEMS];
+    st->max_frontier = std::max<int64_t>(st->max_frontier, (int64_t)ctr[C_MAX_FRONTIER]);
+    st->expand_ms += dp_ms;
+    st->expand_launches += dp_launches;
+    st->sssp_expand_launches += sssp_launches;
+    st->sssp_expand_ms += sssp_ms;
+    st->phase_ms[0] += ph_ms[0];
+    st->phase_ms[1] += ph_ms[1];
+    st->phase_ms[2] += ph_ms[2];
+  }
+  return GST_OK;
+}
+
+// First-fit allocator over an arena of elements; a freed region merges with
+// free neighbours.
+struct Arena {
+  std::map<int64_t, int64_t> free_;  // offset -> size
+  void reset(int64_t n) {
+    free_.clear();
+    if (n > 0) free_[0] = n;
+  }
+  int64_t take(int64_t n) {
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        const int64_t off = it->first, rest = it->second - n;
+        free_.erase(it);
+        if (rest) free_[off + n] = rest;
+        return off;
+      }
+    return -1;
+  }
+  void give(int64_t off, int64_t n) {
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+};
+
+// Device bytes a query holds for the whole of a rolling run: one-label far
+// keys, distances and group masks per vertex, route tables, group distances,
+// best word, descriptor, band state, tasks, slot-list entries.
+int64_t rolling_query_bytes(const gst_graph* g) {
+  return (int64_t)g->Vp * 4 * (2 + CROW) + RTABLE * 4 + VV * 4 + 8 + (int64_t)sizeof(SlotDesc) +
+         (int64_t)sizeof(Sched) + 2 * GST_MAX_K * (int64_t)sizeof(Task) + 8;
+}
+
+// Rolling admission: the queries of one mode (plain or hop-bounded, at most
+// MAX_SLOTS) whose state tables do not fit the budget at once. The one-label
+// phase runs for every query first (in passes whose lists fit a share of the
+// budget); each query's distances, route tables and upper bound then stay
+// resident, about 40 bytes per vertex. The DP runs over a pool of table
+// regions cut from one arena: a query is admitted (completion bounds written,
+// pending masks cleared, seeds queued into the frontier the running queries
+// share) as soon as a region fits it, in call order, and collected (tree
+// recovered, region freed) at the first host check after it has converged,
+// which lets the next query in. Convergence is found on the device
+// (k_collect_done): in the unordered DP a query that expanded nothing in the
+// last iteration has nothing queued any more; in the banded DP a query with
+// no live far-list state when the near list has drained.
+// Returns 100 when a worklist overflowed; the queries not collected yet are
+// then in *left for a re-run with full-size lists.
+int solve_rolling(gst_graph* g, const std::vector<QueryIn>& qs, int64_t budget, BatchOut& out,
+                  gst_stats* st, std::vector<QueryIn>* left) {
+
+  const int nslots = (int)qs.size();
+  const int64_t Vp = g->Vp, nch = g->nch();
+  const bool hop = qs[0].D >= 0;
+  const int maxe = g->max_tree_edges;
+  std::vector<SlotDesc> sd(nslots);
+  std::vector<Task> tasks;
+  std::vector<int64_t> first_task(nslots + 1, 0);
+  int64_t need_max = 0, need_all = 0;
+  double rmax = 0;  // pending-mask words per state, largest over the queries
+  for (int s = 0; s < nslots; ++s) {
+    const QueryIn& q = qs[s];
+    SlotDesc d;
+    std::memset(&d, 0, sizeof(d));
+    d.e_off = (int64_t)s * Vp;
+    d.c_off = (int64_t)s * Vp * CROW;
+    d.r_off = (int64_t)s * RTABLE;
+    d.k = q.k;
+    d.full = (1 << q.k) - 1;
+    d.D = q.D;
+    d.nsub = 1 << q.k;
+    d.vv_off = s * VV;
+    d.nrows = rows_of(q.k, q.D);
+    d.mw = mask_words(q.k);
+    sd[s] = d;
+    first_task[s] = (int64_t)tasks.size();
+    for (int i = 0; i < q.k; ++i) tasks.push_back(Task{s, i, q.groups[i], 0});
+    const int64_t need = (int64_t)d.nrows * Vp;
+    need_max = std::max(need_max, need);
+    need_all += need;
+    rmax = std::max(rmax, (double)mask_words(q.k) / (double)(1 << q.k));
+  }
+  first_task[nslots] = (int64_t)tasks.size();
+  g->dp_div_eff = g->dp_div_call >= 0
+                      ? g->dp_div_call
+                      : (nslots >= kAutoBandSlots && !hop ? kAutoBandDiv : 0);
+  g->sssp_band_eff = g->sssp_band_call >= 0 ? g->sssp_band_call : kAutoSsspBand;
+  const bool far_lists = g->dp_div_eff > 0 || g->sssp_band_eff > 0;
+  const int nl = far_lists ? 5 : 3;  // lists: two frontier lists, state list, two far lists
+  // Budget: resident per-query arrays and tree outputs first, then the
+  // one-label lists of a pass (at most 15% of what is left), then the arena
+  // and the DP's lists, which hold 4% of the arena's states as in a lockstep
+  // batch (a rolling run that overflows them restarts its resident queries,
+  // half as many at a time).
+  const double wl_frac = kWorklistFraction;
+  const int64_t per_query = rolling_query_bytes(g);
+  // trees are recovered in pieces of up to 64 queries, whose output rows
+  // take at most 5% of the budget
+  const int orows = (int)std::max<int64_t>(
+      1, std::min<int64_t>(std::min(nslots, 64), budget / 20 / ((int64_t)maxe * 12)));
+  const int64_t avail = budget - per_query * nslots - (int64_t)orows * maxe * 12;
+  int per_pass = nslots;
+  if (g->sssp_slots > 0) per_pass = std::min(per_pass, (int)g->sssp_slots);
+  per_pass = (int)std::max<int64_t>(
+      1, std::min<int64_t>(per_pass, (int64_t)(0.15 * (double)avail) / (nl * 8 * nch + 1)));
+  const int64_t lists1 = nch * per_pass;
+  const double ratio = (double)nch / (double)Vp;  // items per state row
+  const double per_state = 8.0 + (g->bp_call ? 4.0 : 0.0) + 4.0 * rmax;
+  double A = ((double)avail - nl * 8.0 * (double)lists1) / per_state;
+  if (wl_frac * A * ratio > (double)lists1)
+    A = (double)avail / (per_state + nl * 8.0 * wl_frac * ratio);
+  int64_t arena = std::min<int64_t>((int64_t)A, need_all);
+  // A state table from an earlier call within 5% of this size is kept as it
+  // is: freeing and mapping a table of hundreds of GB again costs seconds.
+  {
+    const int64_t held = (int64_t)(g->T.bytes / 8);
+    if (held >= need_max && (double)held >= 0.95 * (double)arena &&
+        (double)held <= 1.05 * (double)arena)
+      arena = std::min(held, need_all);
+  }
+  if (arena < need_max)
+    return fail(GST_ERR_OOM, "a single query exceeds the HBM budget");
+  int64_t cap = std::max<int64_t>(lists1, (int64_t)(wl_frac * (double)arena * ratio)) + 4096;
+  if (g->list_permille > 0)  // no floors, as for a lockstep batch
+    cap = (int64_t)((double)arena * ratio) * g->list_permille / 1000 + 64;
+  if (cap > kListClamp + 4096) cap = kListClamp + 4096;
+  if (g->wl_cap > 0 && (double)g->wl_cap >= 0.95 * (double)cap &&
+      (double)g->wl_cap <= 1.05 * (double)cap && g->wl_cap >= lists1 + 4096)
+    cap = g->wl_cap;  // lists from an earlier call of about this size are kept
+  const int64_t arena_m = (int64_t)(rmax * (double)arena) + 64;
+
+  bool grew = false;
+  HIP_TRY(g->T.ensure((size_t)arena * 8, &grew));
+  if (g->bp_call) HIP_TRY(g->BPs.ensure((size_t)arena * 4, &grew));
+  HIP_TRY(g->Ms.ensure((size_t)arena_m * 4, &grew));
+  bool e_grew = false;
+  HIP_TRY(g->Es.ensure((size_t)nslots * Vp * 4, &e_grew));
+  if (e_grew) HIP_TRY(hipMemsetAsync(g->Es.p, 0, g->Es.bytes, g->stream));
+  grew |= e_grew;
+  HIP_TRY(g->C.ensure((size_t)nslots * Vp * CROW * 4, &grew));
+  HIP_TRY(g->GMs.ensure((size_t)nslots * Vp * 4, &grew));
+  HIP_TRY(g->R.ensure((size_t)nslots * RTABLE * 4, &grew));
+  HIP_TRY(g->VVb.ensure((size_t)nslots * VV * 4, &grew));
+  HIP_TRY(g->bestw.ensure((size_t)nslots * 8, &grew));
+  HIP_TRY(g->slots.ensure((size_t)nslots * sizeof(SlotDesc), &grew));
+  HIP_TRY(g->sched.ensure((size_t)nslots * sizeof(Sched), &grew));
+  HIP_TRY(g->tasks.ensure(tasks.size() * sizeof(Task) + 16, &grew));
+  HIP_TRY(g->rsel.ensure((size_t)(2 * nslots + 16) * 4, &grew));
+  HIP_TRY(g->rbest.ensure((size_t)nslots * 8, &grew));
+  HIP_TRY(g->rtasks.ensure(tasks.size() * sizeof(Task) + 16, &grew));
+  if ((uint64_t)cap * 8 > g->wl0.bytes || (uint64_t)cap * 8 > g->wl1.bytes ||
+      (uint64_t)cap * 8 > g->wls.bytes) {
+    HIP_TRY(g->wl0.ensure((size_t)cap * 8, &grew));
+    HIP_TRY(g->wl1.ensure((size_t)cap * 8, &grew));
+    HIP_TRY(g->wls.ensure((size_t)cap * 8, &grew));
+  }
+  {
+    const int64_t key = ((int64_t)g->dp_div_eff << 32) | ((uint32_t)g->sssp_band_eff << 1) |
+                        (g->bp_call ? 1 : 0);
+    if (key != g->eff_key) {
+      g->eff_key = key;
+      grew = true;
+    }
+  }
+  if (far_lists &&
+      ((uint64_t)cap * 8 > g->far0.bytes || (uint64_t)cap * 8 > g->far1.bytes)) {
+    HIP_TRY(g->far0.ensure((size_t)cap * 8, &grew));
+    HIP_TRY(g->far1.ensure((size_t)cap * 8, &grew));
+  }
+  size_t list_bytes = std::min({g->wl0.bytes, g->wl1.bytes, g->wls.bytes});
+  if (far_lists) list_bytes = std::min({list_bytes, g->far0.bytes, g->far1.bytes});
+  const uint32_t new_cap = (uint32_t)std::min<size_t>(list_bytes / 8, UINT32_MAX - 65536);
+  if (new_cap != g->wl_cap) {
+    g->wl_cap = new_cap;
+    grew = true;
+  }
+  HIP_TRY(g->out_uv.ensure((size_t)orows * maxe * 8, &grew));
+  HIP_TRY(g->out_eid.ensure((size_t)orows * maxe * 4, &grew));
+  HIP_TRY(g->out_n.ensure((size_t)orows * 4, &grew));
+  HIP_TRY(g->out_err.ensure((size_t)orows * 4, &grew));
+  g->out_rows = orows;
+  if (grew) {
+    ++g->gen;
+    g->drop_graphs();
+  }
+  HIP_TRY(hipMemcpyAsync(g->slots.p, sd.data(), nslots * sizeof(SlotDesc), hipMemcpyHostToDevice,
+                         g->stream));
+  HIP_TRY(hipMemcpyAsync(g->tasks.p, tasks.data(), tasks.size() * sizeof(Task),
+                         hipMemcpyHostToDevice, g->stream));
+  DevCtx c = make_ctx(g);
+  {
+    std::vector<Sched> sch(nslots);
+    for (auto& x : sch) {
+      std::memset(&x, 0, sizeof(x));
+      x.thr = x.delta = g->sssp_band_eff > 0 ? g->sssp_band_eff : INF / 2;
+      x.mindef = INF;
+      x.R = INF;
+    }
+    HIP_TRY(hipMemcpyAsync(g->sched.p, sch.data(), nslots * sizeof(Sched), hipMemcpyHostToDevice,
+                           g->stream));
+    HIP_TRY(hipMemcpyAsync(g->iter.as<int32_t>() + 1, &nslots, sizeof(int32_t),
+                           hipMemcpyHostToDevice, g->stream));
+    g->h_nslots[0] = nslots;
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  }
+  ++g->batch_no;
+  HIP_TRY(hipEventRecord(g->ph[0], g->stream));
+  double sssp_ms = 0, dp_ms = 0;
+  int64_t dp_launches = 0, sssp_launches = 0;
+  auto all_left = [&]() {
+    if (!left) return;
+    left->clear();
+    for (int s = 0; s < nslots; ++s) left->push_back(qs[s]);
+  };
+  {
+    const int rc = onelabel_stage(g, c, tasks, nslots, per_pass, &sssp_ms, &sssp_launches);
+    if (rc == 100) all_left();
+    if (rc) return rc;
+  }
+  // ---- the DP over a rolling pool of table regions
+  // upper bounds as the one-label phase left them (a restarted query starts from these)
+  HIP_TRY(hipMemcpyAsync(g->rbest.p, g->bestw.p, (size_t)nslots * 8, hipMemcpyDeviceToDevice,
+                         g->stream));
+  launch_fill_i32(g->cnt.as<int32_t>(), kCntWords, 0, g->stream);
+  launch_fill_i32(g->iter.as<int32_t>() + 2, 1, 0, g->stream);  // far list in use
+  Arena at, am;
+  at.reset(arena);
+  am.reset(arena_m);
+  std::vector<char> state(nslots, 0);  // 0 waiting, 1 resident, 2 collected
+  std::vector<int64_t> mwords(nslots);
+  std::deque<int32_t> waiting;  // queries not admitted yet, in admission order
+  for (int s = 0; s < nslots; ++s) waiting.push_back(s);
+  int32_t* d_sel = g->rsel.as<int32_t>();
+  int32_t* d_done = d_sel + nslots;
+  uint32_t* d_ndone = reinterpret_cast<uint32_t*>(d_sel + 2 * nslots);
+  std::vector<int32_t> sel;
+  std::vector<Task> new_tasks;
+  int64_t rounds = 0, max_res = 0;
+  g->resident = 0;
+  int cap_res = g->max_batch > 0 ? g->max_batch : nslots;
+  uint32_t* cnt = g->cnt.as<uint32_t>();
+  // admits waiting queries, in call order, while their regions fit
+  auto admit = [&]() -> int {
+    sel.clear();
+    new_tasks.clear();
+    while (!waiting.empty() && g->resident < cap_res) {
+      const int s = waiting.front();
+      const int64_t nt = (int64_t)sd[s].nrows * Vp;
+      const int64_t nm = slot_mask_words(g, qs[s].k, qs[s].D);
+      const int64_t ot = at.take(nt);
+      if (ot < 0) break;
+      const int64_t om = am.take(nm);
+      if (om < 0) {
+        at.give(ot, nt);
+        break;
+      }
+      sd[s].t_off = ot;
+      sd[s].m_off = om;
+      mwords[s] = nm;
+      state[s] = 1;
+      sel.push_back(s);
+      for (int64_t t = first_task[s]; t < first_task[s + 1]; ++t) new_tasks.push_back(tasks[t]);
+      ++g->resident;
+      waiting.pop_front();
+    }
+    if (sel.empty()) return GST_OK;
+    ++rounds;
+    max_res = std::max<int64_t>(max_res, g->resident);
+    HIP_TRY(hipMemcpyAsync(g->slots.p, sd.data(), nslots * sizeof(SlotDesc),
+                           hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(d_sel, sel.data(), sel.size() * 4, hipMemcpyHostToDevice, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->rtasks.p, new_tasks.data(), new_tasks.size() * sizeof(Task),
+                           hipMemcpyHostToDevice, g->stream));
+    launch_lb_table(c, (int)sel.size(), g->stream, d_sel);
+    for (int s : sel)  // pending masks start empty
+      launch_fill_i32(reinterpret_cast<int32_t*>(g->Ms.p) + sd[s].m_off, mwords[s], 0, g->stream);
+    launch_dp_seed(c, g->rtasks.as<Task>(), (int)new_tasks.size(), g->goff.as<int64_t>(),
+                   g->gmem.as<int32_t>(), g->wl0.as<uint64_t>(), cnt, g->stream);
+    launch_dp_sched_init(c, (int)sel.size(), g->stream, d_sel);
+    HIP_TRY(hipGetLastError());
+    return GST_OK;
+  };
+  {
+    const int rc = admit();
+    if (rc) return rc;
+  }
+  if (g->resident == 0) return fail(GST_ERR_OOM, "a single query exceeds the HBM budget");
+  HIP_TRY(hipEventRecord(g->ph[2], g->stream));
+  const int mode = hop ? 2 : 1;
+  const bool far_mode = c.far_mode != 0;
+  int pairs = 1;
+  // a host check every second iteration: a converged query leaves, and an
+  // idle query moves to its next band, at most one iteration later (a check
+  // costs tens of microseconds, an iteration of a pool this large milliseconds)
+  const int max_pairs = 1;
+  int64_t total_pairs = 0;
+  std::vector<int32_t> done;
+  while (true) {
+    {
+      const int prc = phase_round(g, c, mode, pairs, &dp_ms, &dp_launches);
+      if (prc) return prc;
+    }
+    total_pairs += pairs;
+    HIP_TRY(hipMemcpyAsync(g->h_cnt, cnt, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->h_cnt + 2, g->ctr.as<uint64_t>() + C_OVERFLOW, sizeof(uint64_t),
+                           hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+    uint64_t overflow;
+    std::memcpy(&overflow, g->h_cnt + 2, sizeof(overflow));
+    if (overflow && g->resident > 1) {
+      // A list overflowed: the resident queries start their DP again, fewer
+      // at a time (half as many may hold tables), from their one-label
+      // results and their upper bounds as the one-label phase left them.
+      if (st) ++st->reruns;
+      std::vector<int32_t> back;
+      for (int s = 0; s < nslots; ++s)
+        if (state[s] == 1) back.push_back(s);
+      for (auto it = back.rbegin(); it != back.rend(); ++it) {
+        const int s = *it;
+        state[s] = 0;
+        at.give(sd[s].t_off, (int64_t)sd[s].nrows * Vp);
+        am.give(sd[s].m_off, mwords[s]);
+        waiting.push_front(s);
+        HIP_TRY(hipMemcpyAsync(g->bestw.as<uint64_t>() + s, g->rbest.as<uint64_t>() + s, 8,
+                               hipMemcpyDeviceToDevice, g->stream));
+        launch_fill_i32(&g->sched.as<Sched>()[s].phase, 1, 0, g->stream);
+      }
+      cap_res = std::max(1, g->resident / 2);
+      g->resident = 0;
+      launch_fill_i32(g->cnt.as<int32_t>(), kCntWords, 0, g->stream);
+      launch_fill_i32(g->iter.as<int32_t>() + 2, 1, 0, g->stream);  // far list in use
+      launch_fill_u64(g->ctr.as<uint64_t>() + C_OVERFLOW, 1, 0, g->stream);
+      {
+        const int rc = admit();
+        if (rc) return rc;
+      }
+      pairs = 1;
+      continue;
+    }
+    if (overflow) {  // a lone query's frontier: the caller re-runs with full-size lists
+      if (left) {
+        left->clear();
+        for (int s = 0; s < nslots; ++s)
+          if (state[s] != 2) left->push_back(qs[s]);
+      }
+      g->resident = 0;
+      return 100;
+    }
+    // Which converged queries to collect: cmode 0 = those that expanded
+    // nothing in the last iteration (nothing can reach them: no far list, or
+    // it is empty), 1 = those a band refill just found without live far-list
+    // states, 2 = every resident query (both lists are empty).
+    int cmode = -1;
+    if (!far_mode) {
+      cmode = g->h_cnt[0] == 0 ? 2 : 0;
+    } else {
+      int32_t fsel = 0;
+      uint32_t fc[2] = {0, 0};
+      HIP_TRY(hipMemcpyAsync(fc, cnt + 2, sizeof(fc), hipMemcpyDeviceToHost, g->stream));
+      HIP_TRY(hipMemcpyAsync(&fsel, g->iter.as<int32_t>() + 2, sizeof(fsel),
+                             hipMemcpyDeviceToHost, g->stream));
+      HIP_TRY(hipStreamSynchronize(g->stream));
+      const uint32_t nfar = fc[fsel & 1];
+      if (nfar == 0) {
+        cmode = g->h_cnt[0] == 0 ? 2 : 0;
+      } else {
+        // Band refills per query: a query whose near-list work is done moves
+        // to its next band now, not when every resident query's band has
+        // drained (a query admitted late would otherwise advance one band
+        // per drain of the whole pool).
+        uint32_t idle = 1;
+        if (g->h_cnt[0] != 0) {
+          launch_fill_i32(reinterpret_cast<int32_t*>(d_ndone), 1, 0, g->stream);
+          launch_idle_count(c, d_ndone, g->stream);
+          HIP_TRY(hipMemcpyAsync(&idle, d_ndone, sizeof(idle), hipMemcpyDeviceToHost, g->stream));
+          HIP_TRY(hipStreamSynchronize(g->stream));
+        }
+        if (idle > 0) {
+          launch_far_refill(c, nslots, nfar, g->wl0.as<uint64_t>(), cnt, g->stream,
+                            g->h_cnt[0] != 0);
+          HIP_TRY(hipGetLastError());
+          cmode = 1;
+        }
+      }
+    }
+    if (cmode >= 0) {
+      launch_fill_i32(reinterpret_cast<int32_t*>(d_ndone), 1, 0, g->stream);
+      launch_collect_done(c, cmode, d_done, d_ndone, g->stream);
+      HIP_TRY(hipGetLastError());
+      uint32_t nd = 0;
+      HIP_TRY(hipMemcpyAsync(&nd, d_ndone, sizeof(nd), hipMemcpyDeviceToHost, g->stream));
+      HIP_TRY(hipStreamSynchronize(g->stream));
+      if (nd > 0) {
+        done.resize(nd);
+        HIP_TRY(hipMemcpy(done.data(), d_done, nd * 4, hipMemcpyDeviceToHost));
+        const int crc = collect_results(g, c, qs, done, d_done, out);
+        if (crc) return crc;
+        for (int s : done) {
+          if (state[s] != 1) return fail(GST_ERR_INTERNAL, "rolling admission: slot collected twice");
+          state[s] = 2;
+          at.give(sd[s].t_off, (int64_t)sd[s].nrows * Vp);
+          am.give(sd[s].m_off, mwords[s]);
+          --g->resident;
+        }
+        const int rc = admit();
+        if (rc) return rc;
+      }
+    }
+    if (g->resident == 0) {
+      if (waiting.empty()) break;
+      return fail(GST_ERR_INTERNAL, "rolling admission: no query could be admitted");
+    }
+    if (total_pairs > (int64_t)1 << 20)
+      return fail(GST_ERR_INTERNAL, "frontier iteration limit exceeded");
+    pairs = std::min(pairs * 2, max_pairs);
+  }
+  HIP_TRY(hipEventRecord(g->ph[3], g->stream));
+  std::vector<uint64_t> ctr(C_NUM);
+  HIP_TRY(hipMemcpyAsync(ctr.data(), g->ctr.p, C_NUM * 8, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->resident = 0;
+  float ph_ms[3] = {0, 0, 0};
+  for (int i = 0; i < 3; ++i) HIP_TRY(hipEventElapsedTime(&ph_ms[i], g->ph[i], g->ph[i + 1]));
+  if (st) {
+    st->batches += rounds;
+    st->rolling_queries += nslots;
+    st->max_resident = std::max<int64_t>(st->max_resident, max_res);
     st->queries += nslots;
     st->sssp_iterations += (int64_t)ctr[C_SSSP_ITERS];
     st->dp_iterations += (int64_t)ctr[C_DP_ITERS];
@@ -831,12 +1379,71 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
   // stable order: plain queries first, then hop queries (one mode per batch)
   std::stable_sort(qs.begin(), qs.end(),
                    [](const QueryIn& a, const QueryIn& b) { return (a.D >= 0) < (b.D >= 0); });
+  // A batch whose worklists overflowed is solved again with worklists as
+  // large as the state count, which cannot overflow; they are (4 lists x 8
+  // bytes per state) larger, so it runs in sub-batches that fit the budget at
+  // that size and the 32-bit list capacity.
+  auto rerun_full = [&](const std::vector<QueryIn>& batch) -> int {
+    if (st) ++st->reruns;
+    g->release_batch_buffers();  // the held tables and the larger lists never coexist
+    size_t j = 0;
+    while (j < batch.size()) {
+      std::vector<QueryIn> sub;
+      int64_t u = 0, items = 0;
+      while (j < batch.size()) {
+        const QueryIn& q = batch[j];
+        const int64_t it = slot_items(g, q.k, q.D);
+        const int64_t f = gst_query_footprint(g, q.k, q.D) + 32 * it +
+                          (g->bp_call ? bp_bytes(g, q.k, q.D) : 0);
+        if (!sub.empty() && (u + f > budget || items + it > kListClamp)) break;
+        u += f;
+        items += it;
+        sub.push_back(q);
+        ++j;
+      }
+      const int rc = solve_lockstep(g, sub, 1.0, out, st);
+      if (rc == 100)  // only a query with more states than the list can address
+        return fail(GST_ERR_OOM, "query " + std::to_string(sub[0].idx) +
+                                     ": its frontier exceeds the 2^32-item worklist");
+      if (rc) return rc;
+    }
+    return GST_OK;
+  };
+  const size_t cap_slots = g->max_batch > 0 ? std::min<size_t>(g->max_batch, MAX_SLOTS) : MAX_SLOTS;
   size_t i = 0;
   while (i < qs.size()) {
+    const bool hop = qs[i].D >= 0;
+    // Rolling admission when the queries of this mode do not fit one batch.
+    if (g->rolling && !g->noalloc) {
+      size_t j = i;
+      int64_t tot = 0, fmax = 0;
+      while (j < qs.size() && (qs[j].D >= 0) == hop) {
+        const int64_t f = gst_query_footprint(g, qs[j].k, qs[j].D) +
+                          (g->bp_call ? bp_bytes(g, qs[j].k, qs[j].D) : 0);
+        tot += f;
+        fmax = std::max(fmax, f);
+        ++j;
+      }
+      // the queries of a rolling run keep their one-label results resident
+      // until their DP ends: at most half of the budget goes to those, and
+      // room for two of the largest queries stays
+      const int64_t per_run = std::min<int64_t>(
+          MAX_SLOTS, std::min(budget / 2, budget - 2 * fmax) / rolling_query_bytes(g));
+      if ((tot > budget || j - i > cap_slots) && per_run >= 2) {
+        while (i < j) {
+          const size_t e = std::min(j, i + (size_t)per_run);
+          std::vector<QueryIn> chunk(qs.begin() + i, qs.begin() + e);
+          std::vector<QueryIn> left;
+          int rc = solve_rolling(g, chunk, budget, out, st, &left);
+          if (rc == 100) rc = rerun_full(left);
+          if (rc) return rc;
+          i = e;
+        }
+        continue;
+      }
+    }
     std::vector<QueryIn> batch;
     int64_t used = 0;
-    const bool hop = qs[i].D >= 0;
-    const size_t cap_slots = g->max_batch > 0 ? std::min<size_t>(g->max_batch, MAX_SLOTS) : MAX_SLOTS;
     while (i < qs.size() && (qs[i].D >= 0) == hop && batch.size() < cap_slots) {
       const int64_t f = gst_query_footprint(g, qs[i].k, qs[i].D) +
                         (g->bp_call ? bp_bytes(g, qs[i].k, qs[i].D) : 0);
@@ -848,36 +1455,7 @@ int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& 
       ++i;
     }
     int rc = solve_lockstep(g, batch, frac, out, st);
-    if (rc == 100) {
-      // worklists as large as the state count cannot overflow; they are
-      // (4 lists x 8 bytes per state) larger, so the batch is re-run in
-      // sub-batches that fit the budget at that size
-      if (st) ++st->reruns;
-      g->release_batch_buffers();  // the held tables and the larger lists never coexist
-      // Sub-batches are bounded by the budget and by the 32-bit list
-      // capacity: their lists hold every state (and chunk) of every query.
-      size_t j = 0;
-      rc = GST_OK;
-      while (j < batch.size() && rc == GST_OK) {
-        std::vector<QueryIn> sub;
-        int64_t u = 0, items = 0;
-        while (j < batch.size()) {
-          const QueryIn& q = batch[j];
-          const int64_t it = slot_items(g, q.k, q.D);
-          const int64_t f = gst_query_footprint(g, q.k, q.D) + 32 * it +
-                            (g->bp_call ? bp_bytes(g, q.k, q.D) : 0);
-          if (!sub.empty() && (u + f > budget || items + it > kListClamp)) break;
-          u += f;
-          items += it;
-          sub.push_back(q);
-          ++j;
-        }
-        rc = solve_lockstep(g, sub, 1.0, out, st);
-        if (rc == 100)  // only a query with more states than the list can address
-          return fail(GST_ERR_OOM, "query " + std::to_string(sub[0].idx) +
-                                       ": its frontier exceeds the 2^32-item worklist");
-      }
-    }
+    if (rc == 100) rc = rerun_full(batch);
     if (rc == 100) return fail(GST_ERR_INTERNAL, "worklist overflow at full capacity");
     if (rc) return rc;
   }
@@ -1028,7 +1606,7 @@ int upload_graph(gst_graph* g, const I* beg, const I* csr, const I* wgt, int64_t
 
 extern "C" {
 
-int32_t gst_version(void) { return 1; }
+int32_t gst_version(void) { return 2; }
 
 const char* gst_last_error(void) { return g_last_error.c_str(); }
 
@@ -1217,6 +1795,10 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
     case GST_OPT_BACKPOINTERS:
       if (v < -1 || v > 1) return fail(GST_ERR_INVALID, "backpointer mode is -1, 0 or 1");
       g->bp_mode = (int32_t)v;
+      break;
+    case GST_OPT_ROLLING:
+      if (v < 0 || v > 1) return fail(GST_ERR_INVALID, "rolling admission is 0 or 1");
+      g->rolling = (int32_t)v;
       break;
     case GST_OPT_DP_DELTA_DIV:
       if (v < -1 || v > (1 << 20)) return fail(GST_ERR_INVALID, "band divisor out of range");
@@ -1633,7 +2215,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* 
 void gst_graph_destroy(gst_graph* g) {
   if (!g) return;
   g->drop_graphs();
-  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->Ms, &g->wls, &g->GMs, &g->far0, &g->far1,
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->Ms, &g->wls, &g->GMs, &g->far0, &g->far1, &g->rsel, &g->rtasks, &g->rbest,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
                   &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();

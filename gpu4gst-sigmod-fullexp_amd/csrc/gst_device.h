@@ -115,7 +115,14 @@ struct Sched {
   // tree cost the int32 cost range cannot hold): such a query that ends
   // without a solution is reported as out of range, not as infeasible
   int32_t sat;
-  int32_t pad;
+  // Rolling admission (a call whose tables do not all fit at once): 1 while
+  // the slot's query is in the DP phase, 2 once it has converged; `last` = 1
+  // if the slot expanded an item in the last iteration (k_advance), `hasfar`
+  // = 1 if it still had live far-list states at the last band refill.
+  int32_t phase;
+  int32_t last;
+  int32_t hasfar;
+  int32_t pad[2];
 };
 
 struct DevCtx {
@@ -217,10 +224,23 @@ void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float
                      float* gw, int64_t nund, hipStream_t st);
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st);
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st);
-void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st);
-void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st);
+// With `sel` (device, n entries) only the slots listed there; else slots 0..n-1.
+void launch_lb_table(const DevCtx& c, int n, hipStream_t st, const int32_t* sel = nullptr);
+void launch_dp_sched_init(const DevCtx& c, int n, hipStream_t st, const int32_t* sel = nullptr);
+// Rolling admission: lists the DP-phase slots whose query has converged in
+// out[0 .. *out_n) and marks them converged. mode 0: slots that expanded
+// nothing in the last iteration (unordered DP: nothing can reach them any
+// more); 1: slots with no live far-list state at the band refill just made
+// (banded DP, run only when the near list had drained); 2: every DP-phase
+// slot (both lists are empty).
+void launch_collect_done(const DevCtx& c, int mode, int32_t* out, uint32_t* out_n,
+                         hipStream_t st);
+// selective: only slots with no near-list work left (nothing expanded in the
+// last iteration) move to their next band.
 void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
-                       uint32_t* near_cnt, hipStream_t st);
+                       uint32_t* near_cnt, hipStream_t st, bool selective = false);
+// *out += the DP-phase slots that expanded nothing in the last iteration.
+void launch_idle_count(const DevCtx& c, uint32_t* out, hipStream_t st);
 void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
                             hipStream_t st);
 void launch_band_start(const DevCtx& c, hipStream_t st);

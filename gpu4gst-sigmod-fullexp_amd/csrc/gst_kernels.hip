@@ -755,6 +755,7 @@ __global__ void k_advance(DevCtx c, uint32_t* cnt_consumed, const uint32_t* cnt_
     // near/far modes move bands only when the near list has drained (k_far_*)
     const bool fixed = dp ? c.far_mode != 0 : c.sssp_far != 0;
     if (!fixed && !x.active && x.mindef < INF) x.thr = x.mindef + x.delta;
+    x.last = x.active;
     x.active = 0;
     if (!fixed) x.mindef = INF;
   }
@@ -880,10 +881,11 @@ void launch_clamp(const DevCtx& c, int nslots, hipStream_t st) {
   hipLaunchKernelGGL(k_clamp, grid, 256, 0, st, c);
 }
 
-// DP band width from the upper bound known after the one-label phase.
-__global__ void k_dp_sched_init(DevCtx c) {
-  const int ns = *c.nslots;
-  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+// DP band width from the upper bound known after the one-label phase; the
+// slot enters the DP phase.
+__global__ void k_dp_sched_init(DevCtx c, int n, const int32_t* sel) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int s = sel ? sel[i] : i;
     const SlotDesc S = c.slots[s];
     const int32_t best = best_of(&c.bestw[s]);
     int32_t delta = INF / 2;
@@ -895,11 +897,44 @@ __global__ void k_dp_sched_init(DevCtx c) {
     x.thr = delta;
     x.active = 0;
     x.mindef = INF;
+    x.phase = 1;
+    x.last = 1;  // not converged before it has run
+    x.hasfar = 1;
   }
 }
 
-void launch_dp_sched_init(const DevCtx& c, int nslots, hipStream_t st) {
-  hipLaunchKernelGGL(k_dp_sched_init, 1, 256, 0, st, c);
+void launch_dp_sched_init(const DevCtx& c, int n, hipStream_t st, const int32_t* sel) {
+  if (n > 0) hipLaunchKernelGGL(k_dp_sched_init, 1, 256, 0, st, c, n, sel);
+}
+
+__global__ void k_collect_done(DevCtx c, int mode, int32_t* out, uint32_t* out_n) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    Sched& x = c.sched[s];
+    if (x.phase != 1) continue;
+    const bool done = mode == 2 || (mode == 0 ? x.last == 0 : x.hasfar == 0);
+    if (done) {
+      x.phase = 2;
+      out[atomicAdd(out_n, 1u)] = s;
+    }
+  }
+}
+
+void launch_collect_done(const DevCtx& c, int mode, int32_t* out, uint32_t* out_n,
+                         hipStream_t st) {
+  hipLaunchKernelGGL(k_collect_done, 1, 256, 0, st, c, mode, out, out_n);
+}
+
+__global__ void k_idle_count(DevCtx c, uint32_t* out) {
+  const int ns = *c.nslots;
+  for (int s = threadIdx.x; s < ns; s += blockDim.x) {
+    const Sched& x = c.sched[s];
+    if (x.phase == 1 && x.last == 0) atomicAdd(out, 1u);
+  }
+}
+
+void launch_idle_count(const DevCtx& c, uint32_t* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_idle_count, 1, 256, 0, st, c, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -951,10 +986,18 @@ __global__ __launch_bounds__(BLOCK) void k_far_min(DevCtx c) {
 
 // Band widths double at every refill: the first, narrow bands find a tight
 // upper bound early; the later, wide ones finish in few iterations.
-__global__ void k_far_thr(DevCtx c) {
+// selective = 1: only slots that expanded nothing in the last iteration (no
+// near-list work left) move to their next band; the others keep theirs.
+__global__ void k_far_thr(DevCtx c, int selective) {
   const int ns = *c.nslots;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
+    if (selective && x.last != 0) {
+      x.mindef = INF;
+      x.hasfar = 1;
+      continue;
+    }
+    x.hasfar = x.mindef < INF ? 1 : 0;
     if (x.mindef < INF) {
       // a hop-bounded slot starts with no upper bound: its first band is
       // sized from the smallest key (a lower bound of the optimum) instead
@@ -1019,12 +1062,12 @@ __global__ void k_far_flip(DevCtx c) {
 }
 
 void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
-                       uint32_t* near_cnt, hipStream_t st) {
+                       uint32_t* near_cnt, hipStream_t st, bool selective) {
   if (nslots <= 0 || n_far == 0) return;
   const uint32_t blocks = (n_far + BLOCK - 1) / BLOCK;
   const int grid = (int)(blocks < 8192 ? blocks : 8192);
   hipLaunchKernelGGL(k_far_min, grid, BLOCK, 0, st, c);
-  hipLaunchKernelGGL(k_far_thr, 1, 256, 0, st, c);
+  hipLaunchKernelGGL(k_far_thr, 1, 256, 0, st, c, selective ? 1 : 0);
   hipLaunchKernelGGL(k_far_move, grid, BLOCK, 0, st, c, near, near_cnt);
   hipLaunchKernelGGL(k_far_flip, 1, 1, 0, st, c);
 }
@@ -1186,7 +1229,7 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 #ifndef GST_LB_VPL
 #define GST_LB_VPL 2
 #endif
-__global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
+__global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) {
   // A workgroup stages its slot's route tables in LDS (entries for groups
   // 0..6 only: 49 pair routes + 7 row minima per subset) and then walks tiles
   // of 128 consecutive vertices, subsets in passes of TP (a smaller tile than
@@ -1206,7 +1249,7 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   constexpr int TS = TP + 1;
   constexpr int RC = 57;  // compact route row: [i * 7 + j] for i, j < 7, then [49 + i]
   __shared__ int32_t tile[TV * TS];
-  const int s = blockIdx.y;
+  const int s = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   const SlotDesc S = c.slots[s];
   __shared__ int32_t s_r[128 * RC];
   for (int x = threadIdx.x; x < S.nsub * RC; x += BLOCK) {
@@ -1295,14 +1338,14 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c) {
   }
 }
 
-void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st) {
-  if (nslots <= 0) return;  // must run for every batch: it initialises the state words
+void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st, const int32_t* sel) {
+  if (nslots <= 0) return;  // must run for every slot: it initialises the state words
   // enough workgroups to fill the device for one slot as for a thousand
   const int tiles = (c.V + WAVE * GST_LB_VPL - 1) / (WAVE * GST_LB_VPL);
   int per_slot = (8192 + nslots - 1) / nslots;
   if (per_slot < 4) per_slot = 4;
   dim3 grid(tiles < per_slot ? tiles : per_slot, nslots);
-  hipLaunchKernelGGL(k_lb_table, grid, BLOCK, 0, st, c);
+  hipLaunchKernelGGL(k_lb_table, grid, BLOCK, 0, st, c, sel);
 }
 
 // ---------------------------------------------------------------------------

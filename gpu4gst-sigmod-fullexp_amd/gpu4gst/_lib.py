@@ -20,7 +20,7 @@ OK, ERR_INVALID, ERR_OOM, ERR_DEVICE, ERR_CAPACITY, ERR_INTERNAL = 0, 1, 2, 3, 4
 OPT = dict(chunk=1, grid=2, iters_per_sync=3, hop_prune=4, profile=5, infeasible_cost=6,
            max_tree_edges=7, use_graphs=8, sssp_delta=10, dp_delta_div=11, radius_pct=12, max_batch=14,
            sssp_slots=15, list_permille=16, lane_appends=17, per_item_edges=18,
-           backpointers=19, trace=20)
+           backpointers=19, trace=20, rolling=21)
 
 
 class GstStats(C.Structure):
@@ -33,7 +33,8 @@ class GstStats(C.Structure):
         ("sssp_expand_ms", C.c_double), ("phase_ms", C.c_double * 3),
         ("sssp_expand_launches", C.c_int64), ("reruns", C.c_int64), ("ub_rows", C.c_int64),
         ("bytes_layout", C.c_int64), ("backpointers_stored", C.c_int64),
-        ("bp_fallbacks", C.c_int64)]
+        ("bp_fallbacks", C.c_int64), ("rolling_queries", C.c_int64),
+        ("max_resident", C.c_int64)]
 
     def as_dict(self):
         d = {n: getattr(self, n) for n, _ in self._fields_}
@@ -71,6 +72,8 @@ def lib():
             raise RuntimeError(f"libgst.so lacks symbols {missing}")
         vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
         L.gst_version.restype = i32
+        if L.gst_version() < 2:  # gst_stats gained fields in version 2
+            raise RuntimeError(f"{LIB_PATH} is version {L.gst_version()}; this wrapper needs 2")
         L.gst_last_error.restype = C.c_char_p
         L.gst_graph_create.argtypes = [vp, vp, vp, i64, i64, i32, C.POINTER(vp)]
         L.gst_graph_set_edge_ids.argtypes = [vp, vp, i64]

@@ -68,7 +68,8 @@ typedef struct gst_graph gst_graph;
 
 typedef struct gst_stats {
   int64_t queries;
-  int64_t batches;            /* admitted batches (HBM budget rounds) */
+  int64_t batches;            /* admission rounds: lockstep batches, plus (rolling admission)
+                                 host checks that admitted queries */
   int64_t sssp_iterations;    /* frontier iterations of the one-label phase */
   int64_t dp_iterations;      /* frontier iterations of the DP phase */
   int64_t sssp_items;         /* one-label states expanded */
@@ -99,6 +100,8 @@ typedef struct gst_stats {
   int64_t backpointers_stored; /* 1 if the last batch stored backpointers */
   int64_t bp_fallbacks;       /* tree-recovery steps whose stored backpointer did not match
                                  the final table (a racing write) and were searched instead */
+  int64_t rolling_queries;    /* queries solved under rolling admission (GST_OPT_ROLLING) */
+  int64_t max_resident;       /* most queries holding DP tables at once under rolling admission */
 } gst_stats;
 
 /* options for gst_set_option */
@@ -134,6 +137,11 @@ typedef struct gst_stats {
                                       recomputed (memory is tight) */
 #define GST_OPT_TRACE 20           /* 1 = with GST_OPT_PROFILE, record one gst_trace_row per
                                       expand launch (read with gst_trace_get) */
+#define GST_OPT_ROLLING 21         /* 1 (default) = rolling admission for the queries of a mode
+                                      whose tables do not fit the HBM budget at once: the one-label
+                                      phase runs for all of them first, then DP table regions are
+                                      handed to waiting queries as soon as a query converges;
+                                      0 = lockstep batches (each waits for its slowest query) */
 /* Ablation switches (the reference's TrimCDP-WB / D-TrimCDP-WB ablation variants, README.md:
  * 181-302); results are the same, only the speed changes: */
 #define GST_OPT_LANE_APPENDS 17    /* 1 = every lane appends its frontier items with its own
@@ -181,11 +189,15 @@ int gst_solve(gst_graph* g, const int32_t* query_groups, int32_t k, int32_t D, i
               int32_t* out_status, gst_stats* out_stats);
 
 /* nq queries in one call: query q is q_groups[q_offsets[q] .. q_offsets[q+1]),
- * with hop bound q_D[q] (q_D may be NULL = all plain). Queries are admitted in
- * batches whose device footprint fits hbm_budget_bytes (0 = 80% of the free
- * HBM plus what the graph's per-batch buffers already hold) and solved
- * concurrently. A batch whose worklists overflow is solved again in
- * sub-batches with worklists as large as the state count. Trees go to
+ * with hop bound q_D[q] (q_D may be NULL = all plain). Queries run
+ * concurrently within hbm_budget_bytes (0 = 80% of the free HBM plus what the
+ * graph's per-batch buffers already hold): when all queries of a mode fit, as
+ * one lockstep batch; otherwise under rolling admission (GST_OPT_ROLLING): the
+ * one-label phase of all of them first, then DP table regions handed to
+ * waiting queries as queries converge (after a worklist overflow the resident
+ * queries restart, half as many at a time). A lockstep batch whose worklists
+ * overflow is solved again in sub-batches with worklists as large as the state
+ * count. Trees go to
  * out_edge_uv (pairs) / out_edge_ids
  * (undirected ids, may be NULL), query q at [out_edge_offsets[q],
  * out_edge_offsets[q+1]). If edge_cap is too small GST_ERR_CAPACITY is

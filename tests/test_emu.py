@@ -49,6 +49,21 @@ def test_emulated_error_and_range_handling():
     assert out["fails"] == [], out["fails"]
 
 
+def test_emulated_rolling_admission():
+    """Calls larger than the resident cap or the HBM budget run under rolling
+    admission with the oracle's costs and the lockstep batches' results, also
+    when worklists overflow (tests/emu_rolling.py)."""
+    _need_emu()
+    env = dict(os.environ, GPU4GST_LIB=EMU)
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "emu_rolling.py")],
+                       env=env, capture_output=True, text=True, timeout=900)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["cases"] == 24
+    assert out["rolling"] >= 12 and out["restarts"] >= 1, out
+    assert out["fails"] == [], out["fails"]
+
+
 def _adapter_demo(tmp_path):
     exe = tmp_path / "adapter_demo"
     cc = subprocess.run(["g++", "-O1", "-std=c++17", "-I", os.path.join(ROOT, "include"),
