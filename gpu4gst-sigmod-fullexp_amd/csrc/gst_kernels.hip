@@ -127,6 +127,9 @@ __device__ inline void stage_append(uint64_t* stage, uint32_t& n, uint32_t cnt, 
     if (cnt) lane_append(cnt, item, out, out_cnt, cap, ctr);
     return;
   }
+  // most calls append nothing (a relaxation rarely improves its target):
+  // one ballot instead of the wave scan
+  if (__ballot(cnt != 0) == 0) return;
   const int lane = lane_id();
   uint32_t incl = cnt;
 #pragma unroll
@@ -145,54 +148,6 @@ __device__ inline void stage_append(uint64_t* stage, uint32_t& n, uint32_t cnt, 
   for (uint32_t i = 0; i < cnt; ++i) stage[n + excl + i] = item | ((uint64_t)i << 36);
   n += total;
 }
-
-// Appends, for each lane, EPL entries of `cnt[q]` chunk items
-// (items[q] | chunk << 36) with ONE global atomicAdd per workgroup: waves
-// reserve their share of the workgroup's range with an LDS atomic, one
-// thread reserves the range in the list. Every thread of the workgroup must
-// call this together (it synchronises twice); `round` alternates the LDS
-// accumulator so a fast wave's next call cannot meet a stale one.
-__device__ inline void block_append(const uint32_t* cnt, const uint64_t* items, uint64_t* out,
-                                    uint32_t* out_cnt, uint32_t cap, unsigned long long* ctr,
-                                    uint32_t* s_acc, uint32_t* s_base, int round,
-                                    bool per_lane = false) {
-  if (per_lane) {  // uniform over the workgroup
-#pragma unroll
-    for (int q = 0; q < EPL; ++q)
-      if (cnt[q]) lane_append(cnt[q], items[q], out, out_cnt, cap, ctr);
-    return;
-  }
-  const int lane = lane_id();
-  uint32_t mine = 0;
-#pragma unroll
-  for (int q = 0; q < EPL; ++q) mine += cnt[q];
-  uint32_t incl = mine;
-#pragma unroll
-  for (int off = 1; off < WAVE; off <<= 1) {
-    const uint32_t y = __shfl_up(incl, off, WAVE);
-    if (lane >= off) incl += y;
-  }
-  const uint32_t wave_total = __shfl(incl, WAVE - 1, WAVE);
-  uint32_t* acc = &s_acc[round & 1];
-  uint32_t wave_off = 0;
-  if (lane == 0 && wave_total) wave_off = atomicAdd(acc, wave_total);
-  wave_off = __shfl(wave_off, 0, WAVE);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t n = *acc;
-    *s_base = n ? atomicAdd(out_cnt, n) : 0u;
-    s_acc[(round + 1) & 1] = 0;
-  }
-  __syncthreads();
-  uint32_t pos = *s_base + wave_off + (incl - mine);
-#pragma unroll
-  for (int q = 0; q < EPL; ++q)
-    for (uint32_t i = 0; i < cnt[q]; ++i, ++pos) {
-      if (pos < cap) out[pos] = items[q] | ((uint64_t)i << 36);
-      else atomicAdd(&ctr[C_OVERFLOW], 1ull);
-    }
-}
-
 
 __device__ inline void ub_update(uint64_t* bw, int32_t cost, uint32_t wit) {
   const uint64_t pk = ((uint64_t)(uint32_t)cost << 32) | wit;
@@ -528,10 +483,10 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
   __shared__ int4 s_cv[BLOCK][2];      // the item's distances (masked to INF above the radius)
   __shared__ int64_t s_cbase[BLOCK];   // C offset of the item's slot
   __shared__ uint32_t s_meta[BLOCK];   // slot << 8 | k << 1 | plain
-  __shared__ uint32_t s_acc[2], s_abase;  // block_append
+  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];   // frontier appends
   __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded phase)
-  if (threadIdx.x < 2) s_acc[threadIdx.x] = 0;
-  int round = 0;
+  uint64_t* stage = s_stage[threadIdx.x / WAVE];
+  uint32_t st_n = 0;
   const uint32_t n = min(*w.in_cnt, c.cap);  // an overflowed list is never read past its end
   const int32_t cur = *c.iter, next = cur + 1;
   const int tid = threadIdx.x;
@@ -723,11 +678,16 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         cntq[q] = cnt;
         itemq[q] = item;
       }
-      block_append(cntq, itemq, w.out, w.out_cnt, c.cap, c.ctr, s_acc, &s_abase, ++round,
-                   c.lane_appends != 0);
+      // wave-staged appends: no workgroup barrier per round, one list
+      // atomic per WSTAGE items per wave
+#pragma unroll
+      for (int q = 0; q < EPL; ++q)
+        stage_append(stage, st_n, cntq[q], itemq[q], w.out, w.out_cnt, c.cap, c.ctr,
+                     c.lane_appends != 0);
     }
     __syncthreads();
   }
+  stage_flush(stage, st_n, w.out, w.out_cnt, c.cap, c.ctr);
   if (banded)
     stage_flush(fstage, fst_n, c.far[far_sel], c.far_cnt + far_sel, c.cap, c.ctr);
   flush(c.ctr, C_SSSP_ITEMS, n_items);
