@@ -1384,7 +1384,7 @@ void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_
 //     (lb from hop-free distances is a valid bound for hop-bounded trees)
 
 template <bool HOP>
-__global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
+__global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
   __shared__ int32_t s_off[BLOCK + 1];
   __shared__ int32_t s_wsum[BLOCK / WAVE];
   __shared__ int32_t s_beg[BLOCK];
@@ -1656,39 +1656,55 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
     const int32_t total = s_off[BLOCK];
     const bool per_item = c.per_item_edges != 0;
     const int32_t span = per_item ? block_max(deg, s_wsum) * BLOCK : total;
-    // EPL edges per lane per round, all loads issued before any is used
-    for (int32_t base = 0; base < span; base += BLOCK * EPL) {
-      int jq[EPL];
-      int32_t oq[EPL];  // edge offset inside the item's chunk
-      uint32_t uq[EPL];
-      int32_t gq[EPL];
-      int2 dq[EPL];
+    // EPL edges per lane per round, all loads issued before any is used; the
+    // next round's CSR reads are issued before this round's state words are
+    // used, so each round waits on one memory latency, not two
+    int jq[EPL];
+    uint32_t uq[EPL];
+    int32_t wq[EPL];
+    // item, neighbour and weight of the EPL edges of the round at `b`
+    auto fetch = [&](int32_t b, int* jo, uint32_t* uo, int32_t* wo) {
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
+        int32_t o;
         if (per_item) {
-          oq[q] = base / BLOCK + q;
-          jq[q] = oq[q] < deg ? tid : -1;
+          o = b / BLOCK + q;
+          jo[q] = o < deg ? tid : -1;
         } else {
-          const int32_t e = base + q * BLOCK + tid;
-          jq[q] = e < total ? find_item(s_off, e) : -1;
-          oq[q] = jq[q] >= 0 ? e - s_off[jq[q]] : 0;
+          const int32_t e = b + q * BLOCK + tid;
+          jo[q] = e < total ? find_item(s_off, e) : -1;
+          o = jo[q] >= 0 ? e - s_off[jo[q]] : 0;
+        }
+        uo[q] = 0;
+        wo[q] = 0;
+        if (jo[q] >= 0) {
+          const int32_t ej = s_beg[jo[q]] + o;
+          uo[q] = (uint32_t)c.col[ej];
+          wo[q] = c.wt[ej];
         }
       }
-#pragma unroll
-      for (int q = 0; q < EPL; ++q) {
-        uq[q] = 0;
-        gq[q] = INF;
-        if (jq[q] >= 0) {
-          const int32_t ej = s_beg[jq[q]] + oq[q];
-          uq[q] = (uint32_t)c.col[ej];
-          gq[q] = s_cost[jq[q]] + c.wt[ej];
-        }
-      }
+    };
+    if (span > 0) fetch(0, jq, uq, wq);
+    for (int32_t base = 0; base < span; base += BLOCK * EPL) {
+      int2 dq[EPL];
 #pragma unroll
       for (int q = 0; q < EPL; ++q)
         if (jq[q] >= 0)
           dq[q] = *reinterpret_cast<const int2*>(
               &c.T[2 * (s_dst[jq[q]] + (int64_t)uq[q] * s_rows[jq[q]])]);
+      int jn[EPL];
+      uint32_t un[EPL];
+      int32_t wn[EPL];
+      if (base + BLOCK * EPL < span) {
+        fetch(base + BLOCK * EPL, jn, un, wn);
+      } else {
+#pragma unroll
+        for (int q = 0; q < EPL; ++q) {
+          jn[q] = -1;
+          un[q] = 0;
+          wn[q] = 0;
+        }
+      }
 #pragma unroll
       for (int q = 0; q < EPL; ++q) {
         uint32_t want = 0, fwant = 0;
@@ -1696,7 +1712,7 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         if (jq[q] >= 0) {
           const int j = jq[q];
           const uint32_t u = uq[q];
-          const int32_t gc = gq[q];
+          const int32_t gc = s_cost[j] + wq[q];
           const int64_t dbase = s_dst[j];
           const int64_t uoff = (int64_t)u * s_rows[j];
           int32_t* dst = &c.T[2 * (dbase + uoff)];
@@ -1737,6 +1753,12 @@ __global__ __launch_bounds__(BLOCK) void k_dp_expand(DevCtx c, WL w) {
         stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
         if (far_mode)
           stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr, lane_ap);
+      }
+#pragma unroll
+      for (int q = 0; q < EPL; ++q) {
+        jq[q] = jn[q];
+        uq[q] = un[q];
+        wq[q] = wn[q];
       }
     }
     __syncthreads();
