@@ -267,6 +267,9 @@ inline T __builtin_amdgcn_readfirstlane(T x) { return x; }
 // fibers of a wave run one at a time: a fence orders nothing more, and the
 // wave barrier is a collective like the others
 inline void __builtin_amdgcn_fence(int, const char*) {}
+// a streaming store is a store
+template <class T>
+inline void __builtin_nontemporal_store(T v, T* p) { *p = v; }
 inline void __builtin_amdgcn_wave_barrier() { (void)emu::wait(emu::BALLOT, 1); }
 inline uint64_t __ballot(int p) { return emu::wait(emu::BALLOT, p != 0); }
 inline int __any(int p) { return (int)emu::wait(emu::ANY, p != 0); }

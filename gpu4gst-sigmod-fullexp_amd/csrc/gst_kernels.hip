@@ -1291,7 +1291,10 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
         if (h + pp == 0) continue;  // p = 0 is no state
         const int2 word = make_int2(INF, tile[vl * TS + pp]);
         int64_t ix = t_ix(S, h + pp, v0 + vl);
-        for (int dd = 0; dd < nd; ++dd, ix += S.nsub) T2[ix] = word;
+        // streaming stores: the table is far larger than the caches
+        const long long wv = ((long long)(uint32_t)word.y << 32) | (uint32_t)word.x;
+        for (int dd = 0; dd < nd; ++dd, ix += S.nsub)
+          __builtin_nontemporal_store(wv, reinterpret_cast<long long*>(&T2[ix]));
       }
       __syncthreads();
     }
