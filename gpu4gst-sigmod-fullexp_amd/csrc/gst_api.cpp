@@ -294,9 +294,11 @@ DevCtx make_ctx(gst_graph* g) {
 
 int expand_grid(const gst_graph* g) { return g->grid_eff > 0 ? g->grid_eff : g->cus * 8; }
 
-// The unfold step is light per item (a mask read and a few stores): two
-// workgroups per CU keep its launch cheap when the frontier is small.
-int unfold_grid(const gst_graph* g) { return std::min(expand_grid(g), g->cus * 2); }
+// The unfold step: one LDS search per state item it writes; the hop-bounded
+// DP's vertex items carry many pending states, and eight workgroups per CU
+// hide those latencies (hop DP phase 1,655 -> 1,589 ms on the 1.1M-vertex
+// workload; no change on the plain 168k-vertex one).
+int unfold_grid(const gst_graph* g) { return std::min(expand_grid(g), g->cus * 8); }
 
 // Size-class model of the expand settings, from sweeps on the 168k-vertex /
 // 6.8M-edge power-law workload (chunk 128..8192, grid 2..16 x CUs): a lone
@@ -1353,6 +1355,7 @@ int solve_rolling(gst_graph* g, const std::vector<QueryIn>& qs, int64_t budget, 
 int solve_all(gst_graph* g, std::vector<QueryIn>& qs, int64_t budget, BatchOut& out,
               gst_stats* st) {
   if (!g->goff.p) return fail(GST_ERR_INVALID, "groups not set");
+  g->resident = 0;  // a rolling run that failed may have left its count
   if (budget <= 0) {
     size_t fr = 0, tot = 0;
     HIP_TRY(hipMemGetInfo(&fr, &tot));
