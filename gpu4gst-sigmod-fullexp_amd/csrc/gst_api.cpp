@@ -231,12 +231,17 @@ constexpr int kAutoBandSlots = 32;  // automatic DP bands from this many plain q
 constexpr int kAutoBandDiv = 256;   // first band width = upper bound / 256 (>= 1), doubling
 constexpr int kAutoSsspBand = 16;   // first one-label band: labels below 16, doubling per band
 
-// Pending-mask words per (vertex, depth) of a query of k groups.
-int32_t mask_words(int32_t k) { return k <= 5 ? 1 : (1 << k) / 32; }
+// Mask words per (vertex, depth) of a query of k groups: the pending mask, and
+// for a hop-bounded query of up to 5 groups a finite mask beside it (its merge
+// loop enumerates only the partners already reached).
+bool fin_slot(int32_t k, int32_t D) { return D >= 0 && k <= 5; }
+int32_t mask_words(int32_t k, int32_t D) {
+  return fin_slot(k, D) ? 2 : (k <= 5 ? 1 : (1 << k) / 32);
+}
 
 // Pending-mask words of one slot: one mask per (vertex, depth).
 int64_t slot_mask_words(const gst_graph* g, int32_t k, int32_t D) {
-  return (int64_t)(D < 0 ? 1 : D + 1) * g->Vp * mask_words(k);
+  return (int64_t)(D < 0 ? 1 : D + 1) * g->Vp * mask_words(k, D);
 }
 
 // Table footprint of one slot (int32 elements): state words, one-label far
@@ -714,7 +719,8 @@ int solve_lockstep(gst_graph* g, const std::vector<QueryIn>& qs, double wl_frac,
     d.nsub = 1 << q.k;
     d.vv_off = s * VV;
     d.nrows = (int32_t)rows;
-    d.mw = mask_words(q.k);
+    d.mw = mask_words(q.k, q.D);
+    d.fin = fin_slot(q.k, q.D) ? 1 : 0;
     d.m_off = m_tot;
     t_tot += rows * g->Vp;
     e_tot += g->Vp;
@@ -970,14 +976,15 @@ int solve_rolling(gst_graph* g, const std::vector<QueryIn>& qs, int64_t budget, 
     d.nsub = 1 << q.k;
     d.vv_off = s * VV;
     d.nrows = rows_of(q.k, q.D);
-    d.mw = mask_words(q.k);
+    d.mw = mask_words(q.k, q.D);
+    d.fin = fin_slot(q.k, q.D) ? 1 : 0;
     sd[s] = d;
     first_task[s] = (int64_t)tasks.size();
     for (int i = 0; i < q.k; ++i) tasks.push_back(Task{s, i, q.groups[i], 0});
     const int64_t need = (int64_t)d.nrows * Vp;
     need_max = std::max(need_max, need);
     need_all += need;
-    rmax = std::max(rmax, (double)mask_words(q.k) / (double)(1 << q.k));
+    rmax = std::max(rmax, (double)mask_words(q.k, q.D) / (double)(1 << q.k));
   }
   first_task[nslots] = (int64_t)tasks.size();
   g->dp_div_eff = g->dp_div_call >= 0

@@ -67,12 +67,19 @@ struct SlotDesc {
   int32_t k, full, D, nsub;  // D < 0 = plain problem
   int32_t vv_off;  // group distance matrix [8][8] in the vv array
   int32_t nrows;   // rows per vertex: nsub (plain) or nsub * (D + 1) (hop)
-  int32_t mw;      // pending-mask words per (vertex, depth): nsub / 32 rounded up
-  int32_t pad;
-  int64_t m_off;   // DP pending masks: word m_off + (d * Vp + v) * mw + p / 32, bit p % 32,
-                   // says state (v, p, d) waits for the next iteration; bit 0 of word 0
-                   // (p = 0 is no state) says the vertex item (v, d) is queued
+  int32_t mw;      // mask words per (vertex, depth): nsub / 32 rounded up, or 2 (fin)
+  int32_t fin;     // 1: hop-bounded slot of k <= 5 groups whose mask words come in pairs
+                   // {pending, finite}: bit p of the second word says T(v, p, d) < INF
+  int64_t m_off;   // DP masks: word mask_ix(S, v, d) + p / 32, bit p % 32, says state
+                   // (v, p, d) waits for the next iteration; bit 0 of word 0 (p = 0 is no
+                   // state) says the vertex item (v, d) is queued. The words of all depths
+                   // of a vertex are adjacent.
 };
+
+// Word index (in the mask arena) of the masks of (v, d).
+__host__ __device__ inline int64_t mask_ix(const SlotDesc& S, int64_t v, int64_t d) {
+  return S.m_off + (v * (S.D < 0 ? 1 : S.D + 1) + d) * S.mw;
+}
 
 // Index (in int2 entries of T, or int32 entries of BP) of row `row` of vertex v.
 __host__ __device__ inline int64_t t_ix(const SlotDesc& S, int64_t row, int64_t v) {

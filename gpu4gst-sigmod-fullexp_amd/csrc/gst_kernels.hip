@@ -348,13 +348,26 @@ __device__ inline uint32_t pending_labels(uint32_t w, uint32_t it) {
 // caller then appends the vertex item (v, d) once. A state already pending
 // costs one atomic and queues nothing. k_dp_unfold clears the mask when it
 // lists the states, before any thread of the next iteration marks again.
+// A fin slot's one 64-bit atomic also sets bit p of the finite word (every
+// state is marked when its value first drops below INF).
 __device__ inline bool mark_state(const DevCtx& c, const SlotDesc& S, uint32_t v, uint32_t p,
                                   uint32_t d) {
-  uint32_t* m = c.M + S.m_off + ((int64_t)d * c.Vp + v) * S.mw;
+  uint32_t* m = c.M + mask_ix(S, v, d);
   const uint32_t wi = p >> 5, bit = 1u << (p & 31u);
+  if (S.fin) {
+    const unsigned long long b = (unsigned long long)bit;
+    return (atomicOr(reinterpret_cast<unsigned long long*>(m), b | 1ull | (b << 32)) & 1ull) == 0;
+  }
   if (wi == 0) return (atomicOr(m, bit | 1u) & 1u) == 0;
   if (atomicOr(m + wi, bit) & bit) return false;  // already pending: its vertex is queued
   return (atomicOr(m, 1u) & 1u) == 0;
+}
+
+// Non-empty subsets of `mask` (k <= 5) as a set: bit x is set for x a subset of mask, x != 0.
+__device__ inline uint32_t subsets_of(uint32_t mask) {
+  uint32_t r = 0;
+  for (uint32_t x = mask; x; x = (x - 1u) & mask) r |= 1u << x;
+  return r;
 }
 
 // Plain slots: T(v, full) <- t + T(v, full ^ p), the complement merge a state
@@ -763,9 +776,10 @@ __global__ __launch_bounds__(BLOCK) void k_dp_unfold(DevCtx c, const uint64_t* v
       const uint64_t it = vin[idx];
       const uint32_t v = it_v(it), d = it_d(it);
       const SlotDesc S = c.slots[it_s(it)];
-      uint32_t* m = c.M + S.m_off + ((int64_t)d * c.Vp + v) * S.mw;
+      uint32_t* m = c.M + mask_ix(S, v, d);
       uint32_t w[4] = {0u, 0u, 0u, 0u};
-      for (int i = 0; i < S.mw; ++i) {
+      const int nw = S.fin ? 1 : S.mw;  // a fin slot's finite word stays
+      for (int i = 0; i < nw; ++i) {
         w[i] = m[i];
         m[i] = 0u;
       }
@@ -1502,6 +1516,69 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
       stage_append(fstage, fst_n, far_cnt_l, far_item_l, far_out, far_out_cnt, c.cap, c.ctr,
                    lane_ap);
 
+    // The merge of item (vj, pj, dj) of slot sl (cost tj, its vertex's state row at tb,
+    // mk = full ^ pj) with partner (vj, p2, d2) of cost t2 (t2 < INF, within the
+    // margin): writes T(vj, pj | p2, max(dj, d2)) when the merged tree is cheaper and
+    // can still beat the upper bound, and queues it.
+    auto merge_pair = [&](uint32_t sl, uint32_t vj, uint32_t pj, uint32_t dj, uint32_t mk,
+                          uint32_t p2, uint32_t d2, int32_t tj, int32_t t2, int64_t tb,
+                          uint32_t& want, uint64_t& item, uint32_t& fwant, uint64_t& fitem) {
+      const uint32_t P = pj | p2;
+      const int32_t mc = tj + t2;
+      if (mc >= INF) c.sched[sl].sat = 1;  // a tree cost past the int32 cost range
+      const uint32_t nd = HOP ? max(dj, d2) : 0u;
+      const int64_t drow = HOP ? (int64_t)nd * (int64_t)((mk | pj) + 1) + P : (int64_t)P;
+      int32_t* dst = &c.T[2 * (tb + drow)];
+      const int2 dl = *reinterpret_cast<const int2*>(dst);
+      if (!(mc < dl.x)) return;
+      if (HOP && p2 == mk) {  // a full tree
+        if (mc < atomicMin(dst, mc)) {
+          ++nU;
+          if (c.BP) c.BP[tb + drow] = (int32_t)bp_merge(pj, dj, d2);
+          ub_update(&c.bestw[sl], mc, wit_full(vj, nd));
+        }
+        return;
+      }
+      // Completion bound first: a state that cannot beat best + 2 (hop: best)
+      // lies on no better solution, so its entry is left alone and no atomic
+      // is issued.
+      int32_t best = INF;
+      bool keep = true;
+      if (!HOP || c.hop_prune) {
+        best = best_of(&c.bestw[sl]);
+        keep = dl.y < INF && (best >= INF || mc + dl.y <= (HOP ? best : best + 2));
+      }
+      if (!(keep && mc < atomicMin(dst, mc))) return;
+      ++nU;
+      nLb += (uint32_t)__popc(mk ^ p2);  // the improved state's bound terms
+      const SlotDesc Sj = c.slots[sl];
+      if (c.BP) c.BP[tb + drow] = (int32_t)bp_merge(pj, dj, d2);
+      if (!HOP) {
+        ++nUb;
+        const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)vj * CROW], mk ^ p2, c.sched[sl].R);
+        if (sumc < INF && mc + sumc < best) {
+          ub_update(&c.bestw[sl], mc + sumc, wit_onelabel(vj, P));
+          best = mc + sumc;
+        }
+      }
+      // 2/3 rule (Conditional Tree Merging) for the queue only; a state left
+      // out of the queue is combined with its complement now
+      const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
+      if (!enq) {
+        complement_merge(c, Sj, sl, vj, P, mc);
+        return;
+      }
+      if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[sl].thr) {
+        fwant = 1;  // past the band: wait in the far list
+        fitem = mk_item(vj, P, nd, 0, sl);
+      } else if (mark_state(c, Sj, vj, P, nd)) {
+        want = 1;
+        item = mk_item(vj, 0, nd, 0, sl);
+      }
+    };
+    bool fin_item = false;  // the item's merges walk its finite masks (fin slots)
+    uint32_t fin_mask = 0, fin_nd2 = 0;
+    int64_t fin_base = 0;   // finite word of (v, 0); depth d2 at + 2 d2
     // ---- merges, balanced over the workgroup like the grow edges: item j
     // owns pairs s_off[j] .. s_off[j + 1] - 1; its pair r merges with subset
     // number r / nd2 + 1 of its mask (the bits of that number deposited into
@@ -1531,7 +1608,14 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
                           h(b.z));
         }
         const int32_t nd2 = HOP ? Dq - (int32_t)d + 1 : 1;
-        mcnt = ((1 << __popc(mask)) - (HOP ? 1 : 2)) * nd2;
+        if (HOP && S.fin) {
+          fin_item = true;
+          fin_mask = mask;
+          fin_nd2 = (uint32_t)nd2;
+          fin_base = mask_ix(S, v, 0) + 1;
+        } else {
+          mcnt = ((1 << __popc(mask)) - (HOP ? 1 : 2)) * nd2;
+        }
         s_v[tid] = v;
         s_mlim[tid] = mlim;
         s_mm[tid] = mask | ((uint32_t)nd2 << 8);
@@ -1589,61 +1673,51 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
         const int j = jq[q];
         if (j >= 0 && t2q[q] < INF && t2q[q] <= s_mlim[j]) {
           const uint32_t meta = s_meta[j];
-          const uint32_t sl = (meta >> 12) & 0xFFFu, dj = (meta >> 8) & 0xFu, pj = meta & 0xFFu;
-          const uint32_t mk = s_mm[j] & 0xFFu, p2 = p2q[q], P = pj | p2;
-          const uint32_t vj = s_v[j];
-          const int32_t mc = s_cost[j] + t2q[q];
-          if (mc >= INF) c.sched[sl].sat = 1;  // a tree cost past the int32 cost range
-          const uint32_t nd = HOP ? max(dj, d2q[q]) : 0u;
-          const int64_t drow = HOP ? (int64_t)nd * (int64_t)((mk | pj) + 1) + P : (int64_t)P;
-          int32_t* dst = &c.T[2 * (s_tb[j] + drow)];
-          const int2 dl = *reinterpret_cast<const int2*>(dst);
-          if (mc < dl.x) {
-            if (HOP && p2 == mk) {  // a full tree
-              if (mc < atomicMin(dst, mc)) {
-                ++nU;
-                if (c.BP) c.BP[s_tb[j] + drow] = (int32_t)bp_merge(pj, dj, d2q[q]);
-                ub_update(&c.bestw[sl], mc, wit_full(vj, nd));
-              }
-            } else {
-              // Completion bound first: a state that cannot beat best + 2
-              // (hop: best) lies on no better solution, so its entry is left
-              // alone and no atomic is issued.
-              int32_t best = INF;
-              bool keep = true;
-              if (!HOP || c.hop_prune) {
-                best = best_of(&c.bestw[sl]);
-                keep = dl.y < INF && (best >= INF || mc + dl.y <= (HOP ? best : best + 2));
-              }
-              if (keep && mc < atomicMin(dst, mc)) {
-                ++nU;
-                nLb += (uint32_t)__popc(mk ^ p2);  // the improved state's bound terms
-                const SlotDesc Sj = c.slots[sl];
-                if (c.BP) c.BP[s_tb[j] + drow] = (int32_t)bp_merge(pj, dj, d2q[q]);
-                if (!HOP) {
-                  ++nUb;
-                  const int32_t sumc = sum_c(&c.C[Sj.c_off + (int64_t)vj * CROW], mk ^ p2,
-                                             c.sched[sl].R);
-                  if (sumc < INF && mc + sumc < best) {
-                    ub_update(&c.bestw[sl], mc + sumc, wit_onelabel(vj, P));
-                    best = mc + sumc;
-                  }
-                }
-                // 2/3 rule (Conditional Tree Merging) for the queue only; a state
-                // left out of the queue is combined with its complement now
-                const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
-                if (!enq) complement_merge(c, Sj, sl, vj, P, mc);
-                if (enq) {
-                  if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[sl].thr) {
-                    fwant = 1;  // past the band: wait in the far list
-                    fitem = mk_item(vj, P, nd, 0, sl);
-                  } else if (mark_state(c, Sj, vj, P, nd)) {
-                    want = 1;
-                    item = mk_item(vj, 0, nd, 0, sl);
-                  }
-                }
-              }
+          merge_pair((meta >> 12) & 0xFFFu, s_v[j], meta & 0xFFu, (meta >> 8) & 0xFu,
+                     s_mm[j] & 0xFFu, p2q[q], d2q[q], s_cost[j], t2q[q], s_tb[j], want, item,
+                     fwant, fitem);
+        }
+        stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
+        if (far_mode)
+          stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr, lane_ap);
+      }
+    }
+    // Hop-bounded slots of up to 5 groups (fin): each lane walks the partners
+    // of its own item that are already reached, depth by depth, from the
+    // vertex's finite masks (one cache line); the partners never reached,
+    // nearly all of those the dense enumeration reads, are not visited. A
+    // partner reached after its word was read is merged when it is expanded.
+    if (HOP) {
+      bool act = fin_item;
+      uint32_t d2 = 0, cw = 0;
+      const uint32_t subs = act ? subsets_of(fin_mask) : 0u;
+      if (act) cw = c.M[fin_base] & subs;
+      while (__any(act)) {
+        uint32_t want = 0, fwant = 0;
+        uint64_t item = 0, fitem = 0;
+        if (act) {
+          if (cw) {
+            const uint32_t p2 = (uint32_t)__builtin_ctz(cw);
+            cw &= cw - 1u;
+            const uint4 cv = s_cv[tid];
+            const uint32_t w4[4] = {cv.x, cv.y, cv.z, cv.w};
+            int32_t lbp = 0;
+#pragma unroll
+            for (int i = 0; i < 7; ++i)
+              if ((p2 >> i) & 1u)
+                lbp = max(lbp, (int32_t)((w4[i >> 1] >> (16 * (i & 1))) & 0xFFFFu));
+            const int32_t mlim = s_mlim[tid];
+            if (lbp <= mlim) {
+              ++nRm;
+              const int64_t tb = s_tb[tid];
+              const int32_t t2 = c.T[2 * (tb + (int64_t)d2 * (int32_t)(fin_mask | p) + d2 + p2)];
+              if (t2 < INF && t2 <= mlim)
+                merge_pair(s, v, p, d, fin_mask, p2, d2, t, t2, tb, want, item, fwant, fitem);
             }
+          } else if (++d2 < fin_nd2) {
+            cw = c.M[fin_base + 2 * (int64_t)d2] & subs;
+          } else {
+            act = false;
           }
         }
         stage_append(stage, st_n, want, item, w.out, w.out_cnt, c.cap, c.ctr, lane_ap);
