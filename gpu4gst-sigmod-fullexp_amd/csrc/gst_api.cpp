@@ -232,11 +232,13 @@ constexpr int kAutoBandDiv = 256;   // first band width = upper bound / 256 (>= 
 constexpr int kAutoSsspBand = 16;   // first one-label band: labels below 16, doubling per band
 
 // Mask words per (vertex, depth) of a query of k groups: the pending mask, and
-// for a hop-bounded query of up to 5 groups a finite mask beside it (its merge
-// loop enumerates only the partners already reached).
-bool fin_slot(int32_t k, int32_t D) { return D >= 0 && k <= 5; }
+// for a plain query or a hop-bounded one of up to 5 groups a finite mask
+// interleaved with it word by word (the merge loop enumerates only the
+// partners already reached).
+bool fin_slot(int32_t k, int32_t D) { return D < 0 || k <= 5; }
 int32_t mask_words(int32_t k, int32_t D) {
-  return fin_slot(k, D) ? 2 : (k <= 5 ? 1 : (1 << k) / 32);
+  const int32_t pend = k <= 5 ? 1 : (1 << k) / 32;
+  return fin_slot(k, D) ? 2 * pend : pend;
 }
 
 // Pending-mask words of one slot: one mask per (vertex, depth).

@@ -67,9 +67,10 @@ struct SlotDesc {
   int32_t k, full, D, nsub;  // D < 0 = plain problem
   int32_t vv_off;  // group distance matrix [8][8] in the vv array
   int32_t nrows;   // rows per vertex: nsub (plain) or nsub * (D + 1) (hop)
-  int32_t mw;      // mask words per (vertex, depth): nsub / 32 rounded up, or 2 (fin)
-  int32_t fin;     // 1: hop-bounded slot of k <= 5 groups whose mask words come in pairs
-                   // {pending, finite}: bit p of the second word says T(v, p, d) < INF
+  int32_t mw;      // mask words per (vertex, depth): nsub / 32 rounded up, doubled (fin)
+  int32_t fin;     // 1: mask words come in pairs {pending i, finite i}: bit p % 32 of
+                   // finite word p / 32 says T(v, p, d) < INF (plain slots, and
+                   // hop-bounded slots of k <= 5 groups)
   int64_t m_off;   // DP masks: word mask_ix(S, v, d) + p / 32, bit p % 32, says state
                    // (v, p, d) waits for the next iteration; bit 0 of word 0 (p = 0 is no
                    // state) says the vertex item (v, d) is queued. The words of all depths

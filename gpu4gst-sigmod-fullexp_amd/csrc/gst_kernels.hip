@@ -348,19 +348,61 @@ __device__ inline uint32_t pending_labels(uint32_t w, uint32_t it) {
 // caller then appends the vertex item (v, d) once. A state already pending
 // costs one atomic and queues nothing. k_dp_unfold clears the mask when it
 // lists the states, before any thread of the next iteration marks again.
-// A fin slot's one 64-bit atomic also sets bit p of the finite word (every
-// state is marked when its value first drops below INF).
+// In a fin slot the same 64-bit atomic sets bit p of the finite word beside
+// the pending word (a state is marked when its value first drops below INF,
+// or set_finite is called where it is not queued).
 __device__ inline bool mark_state(const DevCtx& c, const SlotDesc& S, uint32_t v, uint32_t p,
                                   uint32_t d) {
   uint32_t* m = c.M + mask_ix(S, v, d);
   const uint32_t wi = p >> 5, bit = 1u << (p & 31u);
   if (S.fin) {
-    const unsigned long long b = (unsigned long long)bit;
-    return (atomicOr(reinterpret_cast<unsigned long long*>(m), b | 1ull | (b << 32)) & 1ull) == 0;
+    unsigned long long* m2 = reinterpret_cast<unsigned long long*>(m);
+    const unsigned long long b = (unsigned long long)bit | ((unsigned long long)bit << 32);
+    if (wi == 0) return (atomicOr(m2, b | 1ull) & 1ull) == 0;
+    if (atomicOr(m2 + wi, b) & (unsigned long long)bit) return false;  // already pending
+    return (atomicOr(m, 1u) & 1u) == 0;
   }
   if (wi == 0) return (atomicOr(m, bit | 1u) & 1u) == 0;
   if (atomicOr(m + wi, bit) & bit) return false;  // already pending: its vertex is queued
   return (atomicOr(m, 1u) & 1u) == 0;
+}
+
+// Sets the finite bit of a fin slot's state (v, p, d) whose value dropped
+// below INF without it being queued (sent to the far list, or kept out of the
+// queue by the 2/3 rule): it is still a merge partner.
+__device__ inline void set_finite(const DevCtx& c, const SlotDesc& S, uint32_t v, uint32_t p,
+                                  uint32_t d) {
+  if (S.fin) atomicOr(c.M + mask_ix(S, v, d) + 2 * (p >> 5) + 1, 1u << (p & 31u));
+}
+
+// Subsets of `mask` (k <= 7) as a 128-bit set: bit x is set for x a subset of mask.
+__device__ inline uint4 subsets128(uint32_t mask) {
+  uint64_t lo = 1ull, hi = 0ull;  // {0}
+  for (uint32_t b = 0; b < 7; ++b) {
+    if (!((mask >> b) & 1u)) continue;
+    const uint32_t sh = 1u << b;  // every subset so far, with bit b added
+    if (sh == 64) {
+      hi |= lo;
+    } else {
+      hi |= (hi << sh) | (lo >> (64 - sh));
+      lo |= lo << sh;
+    }
+  }
+  return make_uint4((uint32_t)lo, (uint32_t)(lo >> 32), (uint32_t)hi, (uint32_t)(hi >> 32));
+}
+
+// Position of the r-th set bit (r < popcount) of a 128-bit set.
+__device__ inline uint32_t nth_bit128(const uint4& m, uint32_t r) {
+  const uint32_t w[4] = {m.x, m.y, m.z, m.w};
+  uint32_t i = 0;
+  for (; i < 3; ++i) {
+    const uint32_t pc = (uint32_t)__popc(w[i]);
+    if (r < pc) break;
+    r -= pc;
+  }
+  uint32_t x = w[i];
+  for (; r; --r) x &= x - 1u;  // drop the r lowest set bits
+  return 32u * i + (uint32_t)__builtin_ctz(x);
 }
 
 // Non-empty subsets of `mask` (k <= 5) as a set: bit x is set for x a subset of mask, x != 0.
@@ -778,10 +820,11 @@ __global__ __launch_bounds__(BLOCK) void k_dp_unfold(DevCtx c, const uint64_t* v
       const SlotDesc S = c.slots[it_s(it)];
       uint32_t* m = c.M + mask_ix(S, v, d);
       uint32_t w[4] = {0u, 0u, 0u, 0u};
-      const int nw = S.fin ? 1 : S.mw;  // a fin slot's finite word stays
+      // a fin slot's pending words are every second word; its finite words stay
+      const int step = S.fin ? 2 : 1, nw = S.mw / step;
       for (int i = 0; i < nw; ++i) {
-        w[i] = m[i];
-        m[i] = 0u;
+        w[i] = m[i * step];
+        m[i * step] = 0u;
       }
       w[0] &= ~1u;  // the queued flag is no state
       const uint32_t pc = (uint32_t)(__popc(w[0]) + __popc(w[1]) + __popc(w[2]) + __popc(w[3]));
@@ -1414,6 +1457,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
   __shared__ uint32_t s_mm[BLOCK];     //   mask | partner depths << 8
   __shared__ int64_t s_tb[BLOCK];      //   T offset of the vertex's row 0
   __shared__ uint4 s_cv[BLOCK];        //   one-label distances, 16 bits each
+  __shared__ uint4 s_cand[BLOCK];      //   plain: the reached partner subsets
   __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
   __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];  // far-list appends (banded DP)
   uint64_t* stage = s_stage[threadIdx.x / WAVE];
@@ -1565,12 +1609,14 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
       // out of the queue is combined with its complement now
       const bool enq = HOP || (int64_t)1000 * mc <= (int64_t)667 * best + 2000;
       if (!enq) {
+        set_finite(c, Sj, vj, P, nd);
         complement_merge(c, Sj, sl, vj, P, mc);
         return;
       }
       if (far_mode && min(mc + dl.y, INF - 1) >= c.sched[sl].thr) {
         fwant = 1;  // past the band: wait in the far list
         fitem = mk_item(vj, P, nd, 0, sl);
+        set_finite(c, Sj, vj, P, nd);
       } else if (mark_state(c, Sj, vj, P, nd)) {
         want = 1;
         item = mk_item(vj, 0, nd, 0, sl);
@@ -1613,6 +1659,22 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
           fin_mask = mask;
           fin_nd2 = (uint32_t)nd2;
           fin_base = mask_ix(S, v, 0) + 1;
+        } else if (!HOP && S.fin) {
+          // the partners already reached: non-empty proper subsets of the mask
+          // (its complement was merged above) whose finite bit is set, as one
+          // snapshot the pairs below are numbered in
+          const uint32_t* fw = c.M + mask_ix(S, v, 0);
+          const uint4 subs = subsets128(mask);
+          uint32_t cw[4] = {fw[1] & subs.x, 0u, 0u, 0u};
+          if (S.mw > 2) cw[1] = fw[3] & subs.y;
+          if (S.mw > 4) {
+            cw[2] = fw[5] & subs.z;
+            cw[3] = fw[7] & subs.w;
+          }
+          cw[0] &= ~1u;
+          cw[mask >> 5] &= ~(1u << (mask & 31u));
+          s_cand[tid] = make_uint4(cw[0], cw[1], cw[2], cw[3]);
+          mcnt = __popc(cw[0]) + __popc(cw[1]) + __popc(cw[2]) + __popc(cw[3]);
         } else {
           mcnt = ((1 << __popc(mask)) - (HOP ? 1 : 2)) * nd2;
         }
@@ -1650,8 +1712,12 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
             d2q[q] = r - (idx - 1) * nd2;
           }
           uint32_t p2 = 0;
-          for (uint32_t rest = mk; rest; rest &= rest - 1u, idx >>= 1)
-            if (idx & 1u) p2 |= rest & (0u - rest);
+          if (HOP) {
+            for (uint32_t rest = mk; rest; rest &= rest - 1u, idx >>= 1)
+              if (idx & 1u) p2 |= rest & (0u - rest);
+          } else {
+            p2 = nth_bit128(s_cand[j], r);
+          }
           p2q[q] = p2;
           const uint4 cv = s_cv[j];
           const uint32_t w4[4] = {cv.x, cv.y, cv.z, cv.w};
@@ -1820,6 +1886,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
               if (far_mode && min(gc + dl.y, INF - 1) >= c.sched[sl].thr) {
                 fwant = 1;  // past the band: wait in the far list
                 fitem = mk_item(u, pd, HOP ? dd + 1 : 0, 0, sl);
+                set_finite(c, Sj, u, pd, HOP ? dd + 1 : 0);
               } else if (mark_state(c, Sj, u, pd, HOP ? dd + 1 : 0)) {
                 want = 1;
                 item = mk_item(u, 0, HOP ? dd + 1 : 0, 0, sl);
