@@ -158,6 +158,7 @@ struct gst_graph {
   DBuf T, Es, BPs, Ms, GMs, far0, far1, C, R, VVb, bestw, slots, tasks, wl0, wl1, wls, cnt, iter,
       ctr, sched;
   DBuf out_uv, out_eid, out_n, out_err;
+  DBuf outpack;  // tree rows packed to their longest filled prefix for the copy back
   uint32_t* h_cnt = nullptr;  // pinned
   int32_t h_nslots[1] = {0};  // slots of the batch being solved
   // Rolling admission (GST_OPT_ROLLING): a call whose tables do not fit the
@@ -540,7 +541,21 @@ int collect_chunk(gst_graph* g, const DevCtx& c, const std::vector<QueryIn>& qs,
     if (oerr[o] == 0) max_on = std::max(max_on, std::min(on[o], maxe));
   std::vector<int32_t> huv((size_t)n * std::max(max_on, 1) * 2),
       hid((size_t)n * std::max(max_on, 1));
-  if (max_on > 0) {
+  if (max_on > 0 && n > 1 && !g->noalloc) {
+    // rows packed to max_on edges on the device, then one copy each (a 2D copy
+    // of a thousand short rows from device to pageable memory took 14 ms)
+    const size_t w = (size_t)n * max_on;
+    HIP_TRY(g->outpack.ensure(w * 12));
+    int32_t* puv = g->outpack.as<int32_t>();
+    int32_t* pid = puv + 2 * w;
+    launch_pack_rows(g->out_uv.as<int32_t>(), 2 * (int64_t)maxe, puv, 2 * (int64_t)max_on, n,
+                     g->stream);
+    launch_pack_rows(g->out_eid.as<int32_t>(), maxe, pid, max_on, n, g->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(huv.data(), puv, w * 8, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(hid.data(), pid, w * 4, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipStreamSynchronize(g->stream));
+  } else if (max_on > 0) {
     HIP_TRY(hipMemcpy2D(huv.data(), (size_t)max_on * 8, g->out_uv.p, (size_t)maxe * 8,
                         (size_t)max_on * 8, n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy2D(hid.data(), (size_t)max_on * 4, g->out_eid.p, (size_t)maxe * 4,
@@ -2227,7 +2242,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_off, const int32_t* 
 void gst_graph_destroy(gst_graph* g) {
   if (!g) return;
   g->drop_graphs();
-  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->Ms, &g->wls, &g->GMs, &g->far0, &g->far1, &g->rsel, &g->rtasks, &g->rbest,
+  DBuf* bufs[] = {&g->rowp, &g->col, &g->wt, &g->eid, &g->goff, &g->gmem, &g->T, &g->Es, &g->BPs, &g->Ms, &g->wls, &g->GMs, &g->far0, &g->far1, &g->rsel, &g->rtasks, &g->rbest, &g->outpack,
                   &g->C, &g->R, &g->VVb, &g->bestw, &g->slots, &g->tasks, &g->wl0, &g->wl1,
                   &g->cnt, &g->iter, &g->ctr, &g->sched, &g->out_uv, &g->out_eid, &g->out_n, &g->out_err};
   for (DBuf* b : bufs) b->release();

@@ -228,6 +228,9 @@ void launch_dp_seed(const DevCtx& c, const Task* tasks, int ntasks, const int64_
 void launch_backtrack(const DevCtx& c, int nslots, int max_edges, int32_t* out_uv,
                       int32_t* out_eid, int32_t* out_n, int32_t* out_err, const int32_t* sel,
                       hipStream_t st);
+// Copies the first dp ints of each of `rows` rows (source pitch sp) to rows of pitch dp.
+void launch_pack_rows(const int32_t* src, int64_t sp, int32_t* dst, int64_t dp, int rows,
+                      hipStream_t st);
 void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
                      float* gw, int64_t nund, hipStream_t st);
 void launch_init_slot_tables(const DevCtx& c, int nslots, int64_t c_total, hipStream_t st);

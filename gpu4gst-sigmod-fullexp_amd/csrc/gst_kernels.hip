@@ -2246,6 +2246,18 @@ __global__ void k_backward(int nq, const int64_t* off, const int32_t* ids, const
   }
 }
 
+// rows x cols ints from a source of pitch sp to a destination of pitch dp (cols = dp <= sp)
+__global__ void k_pack_rows(const int32_t* src, int64_t sp, int32_t* dst, int64_t dp, int rows) {
+  for (int r = blockIdx.x; r < rows; r += gridDim.x)
+    for (int64_t i = threadIdx.x; i < dp; i += blockDim.x) dst[r * dp + i] = src[r * sp + i];
+}
+
+void launch_pack_rows(const int32_t* src, int64_t sp, int32_t* dst, int64_t dp, int rows,
+                      hipStream_t st) {
+  if (rows <= 0 || dp <= 0) return;
+  hipLaunchKernelGGL(k_pack_rows, rows < 4096 ? rows : 4096, 128, 0, st, src, sp, dst, dp, rows);
+}
+
 void launch_backward(int nq, const int64_t* off, const int32_t* ids, const float* gout,
                      float* gw, int64_t nund, hipStream_t st) {
   if (nq <= 0) return;
