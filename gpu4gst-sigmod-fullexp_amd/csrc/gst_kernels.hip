@@ -625,7 +625,9 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
         const bool more = cut_all < rend && c.wt[cut_all] <= radius - 1 - lo_all;
         if ((beyond || more) && atomicMax(&c.E[S.e_off + v], bkey) < bkey) {
           far_want = 1;
-          far_item = mk_item(v, 0, 0, 0, s);
+          // the chunk field carries the vertex's chunk count less one: the
+          // refill makes one continuation per chunk without reading the row
+          far_item = mk_item(v, 0, 0, nchunks(c, v) - 1u, s);
         }
       }
     }
@@ -1130,10 +1132,10 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* nea
     if (i < n) {
       const uint64_t it = far[i];
       const Sched& x = c.sched[it_s(it)];
-      const uint32_t v = it_v(it);
       if (x.prev < x.thr) {  // the slot's band grew: v continues
-        want = nchunks(c, v);
-        item = it | ((uint64_t)0x80u << 24);  // continuation flag in the subset field
+        want = it_ch(it) + 1u;
+        // continuation flag in the subset field, chunk field cleared for the chunk index
+        item = (it & ~((uint64_t)0xFFFFu << 36)) | ((uint64_t)0x80u << 24);
       }
     }
     stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
