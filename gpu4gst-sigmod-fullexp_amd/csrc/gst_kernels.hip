@@ -1245,6 +1245,7 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 #ifndef GST_LB_TP
 #define GST_LB_TP 16
 #endif
+typedef long long i64x2 __attribute__((vector_size(16)));  // two state words
 #ifndef GST_LB_VPL
 #define GST_LB_VPL 2
 #endif
@@ -1297,7 +1298,10 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
     for (int h = 0; h < S.nsub; h += TP) {
       const int np = min(TP, S.nsub - h);  // a power of two
       for (int p = h + wv; p < h + np; p += BLOCK / WAVE) {
-        if (p == 0) continue;  // p = 0 is no state
+        if (p == 0) {  // p = 0 is no state: its word is written as {INF, 0}
+          for (int r = 0; r < VPL; ++r) tile[(lane + WAVE * r) * TS] = 0;
+          continue;
+        }
         const uint32_t U = (uint32_t)__builtin_amdgcn_readfirstlane((uint32_t)S.full ^ (uint32_t)p);
         const int32_t* R = s_r + U * RC;
         int32_t one[VPL], mn[VPL], lb1[VPL], lb2[VPL];
@@ -1342,18 +1346,22 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
         }
       }
       __syncthreads();
-      // one lane per (vertex, subset); a hop slot's lane writes the word for
-      // every depth d (rows d * 2^k + p), consecutive lanes consecutive rows
-      const int sh = __builtin_ctz((unsigned)np);
+      // one lane per (vertex, two consecutive subsets): a 16-byte store of two
+      // state words (np >= 2, rows and table offsets are even); a hop slot's
+      // lane writes the pair for every depth d (rows d * 2^k + p), consecutive
+      // lanes consecutive pairs
+      const int sh = __builtin_ctz((unsigned)np) - 1;
+      const int64_t tb = t_ix(S, h, v0);
+#pragma unroll 4
       for (int x = threadIdx.x; x < (nv << sh); x += BLOCK) {
-        const int vl = x >> sh, pp = x & (np - 1);
-        if (h + pp == 0) continue;  // p = 0 is no state
-        const int2 word = make_int2(INF, tile[vl * TS + pp]);
-        int64_t ix = t_ix(S, h + pp, v0 + vl);
+        const int vl = x >> sh, pp = 2 * (x & ((np >> 1) - 1));
+        const uint32_t lb0 = (uint32_t)tile[vl * TS + pp], lb1 = (uint32_t)tile[vl * TS + pp + 1];
+        const i64x2 pair = {(long long)(((uint64_t)lb0 << 32) | (uint32_t)INF),
+                            (long long)(((uint64_t)lb1 << 32) | (uint32_t)INF)};
+        int64_t ix = tb + (int64_t)vl * nrows + pp;
         // streaming stores: the table is far larger than the caches
-        const long long wv = ((long long)(uint32_t)word.y << 32) | (uint32_t)word.x;
         for (int dd = 0; dd < nd; ++dd, ix += S.nsub)
-          __builtin_nontemporal_store(wv, reinterpret_cast<long long*>(&T2[ix]));
+          __builtin_nontemporal_store(pair, reinterpret_cast<i64x2*>(&T2[ix]));
       }
       __syncthreads();
     }
