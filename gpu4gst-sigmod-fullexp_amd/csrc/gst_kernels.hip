@@ -970,10 +970,17 @@ __device__ inline int64_t item_row(const SlotDesc& S, uint64_t it) {
 
 // *half_drop (may be null) is set when a plain state is dropped by the 1/2
 // rule alone: the caller owes it its complement merge.
-__device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f,
+// The state word of far item `it`.
+__device__ inline int2 far_word(const DevCtx& c, uint64_t it) {
+  const SlotDesc S = c.slots[it_s(it)];
+  return *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
+}
+
+// `tl` is the item's state word (far_word), read by the caller so that the
+// reads of several items are in flight together.
+__device__ inline bool far_live(const DevCtx& c, uint64_t it, int2 tl, int32_t* f,
                                 bool* half_drop = nullptr) {
   const SlotDesc S = c.slots[it_s(it)];
-  const int2 tl = *reinterpret_cast<const int2*>(&c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
   const int32_t t = tl.x;
   if (t >= INF) return false;
   const int32_t best = best_of(&c.bestw[it_s(it)]);
@@ -989,16 +996,32 @@ __device__ inline bool far_live(const DevCtx& c, uint64_t it, int32_t* f,
   return true;
 }
 
+// The refill kernels take FPL far items per lane per round and issue all
+// their reads before using any (each item's state word is a random read).
+constexpr int FPL = 4;
+
 __global__ __launch_bounds__(BLOCK) void k_far_min(DevCtx c) {
   const int sel = *c.far_sel;
   const uint32_t n = min(c.far_cnt[sel], c.cap);
   const uint64_t* far = c.far[sel];
-  for (uint32_t i = blockIdx.x * BLOCK + threadIdx.x; i < n; i += gridDim.x * BLOCK) {
-    const uint64_t it = far[i];
-    int32_t f;
-    if (far_live(c, it, &f)) {
-      int32_t* m = &c.sched[it_s(it)].mindef;
-      if (f < *m) atomicMin(m, f);
+  for (uint32_t base = blockIdx.x * BLOCK * FPL; base < n; base += gridDim.x * BLOCK * FPL) {
+    uint64_t itq[FPL];
+    int2 tlq[FPL];
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      const uint32_t i = base + q * BLOCK + threadIdx.x;
+      itq[q] = i < n ? far[i] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < FPL; ++q)
+      if (base + q * BLOCK + threadIdx.x < n) tlq[q] = far_word(c, itq[q]);
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      int32_t f;
+      if (base + q * BLOCK + threadIdx.x < n && far_live(c, itq[q], tlq[q], &f)) {
+        int32_t* m = &c.sched[it_s(itq[q])].mindef;
+        if (f < *m) atomicMin(m, f);
+      }
     }
   }
 }
@@ -1038,37 +1061,49 @@ __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, ui
   const uint64_t* far = c.far[sel];
   uint64_t* far_out = c.far[1 - sel];
   uint32_t* far_out_cnt = c.far_cnt + (1 - sel);
-  for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-    const uint32_t i = base + threadIdx.x;
-    uint32_t want = 0, fwant = 0;
-    uint64_t item = 0, fitem = 0;
-    if (i < n) {
-      const uint64_t it = far[i];
-      int32_t f;
-      bool half_drop = false;
-      const bool live = far_live(c, it, &f, &half_drop);
-      if (half_drop) {  // never expanded now: combine it with its complement first
-        const SlotDesc S = c.slots[it_s(it)];
-        complement_merge(c, S, it_s(it), it_v(it), it_p(it),
-                         c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
-      }
-      if (live) {
-        const uint32_t v = it_v(it), s = it_s(it);
-        const SlotDesc S = c.slots[s];
-        if (f < c.sched[s].thr) {
-          // back to the frontier; duplicates of a state in the far list collapse here
-          if (mark_state(c, S, v, it_p(it), it_d(it))) {
-            want = 1;
-            item = mk_item(v, 0, it_d(it), 0, s);
+  for (uint32_t base = blockIdx.x * BLOCK * FPL; base < n; base += gridDim.x * BLOCK * FPL) {
+    uint64_t itq[FPL];
+    int2 tlq[FPL];
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      const uint32_t i = base + q * BLOCK + threadIdx.x;
+      itq[q] = i < n ? far[i] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < FPL; ++q)
+      if (base + q * BLOCK + threadIdx.x < n) tlq[q] = far_word(c, itq[q]);
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      uint32_t want = 0, fwant = 0;
+      uint64_t item = 0, fitem = 0;
+      if (base + q * BLOCK + threadIdx.x < n) {
+        const uint64_t it = itq[q];
+        int32_t f;
+        bool half_drop = false;
+        const bool live = far_live(c, it, tlq[q], &f, &half_drop);
+        if (half_drop) {  // never expanded now: combine it with its complement first
+          const SlotDesc S = c.slots[it_s(it)];
+          complement_merge(c, S, it_s(it), it_v(it), it_p(it),
+                           c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
+        }
+        if (live) {
+          const uint32_t v = it_v(it), s = it_s(it);
+          const SlotDesc S = c.slots[s];
+          if (f < c.sched[s].thr) {
+            // back to the frontier; duplicates of a state in the far list collapse here
+            if (mark_state(c, S, v, it_p(it), it_d(it))) {
+              want = 1;
+              item = mk_item(v, 0, it_d(it), 0, s);
+            }
+          } else {
+            fwant = 1;
+            fitem = it;
           }
-        } else {
-          fwant = 1;
-          fitem = it;
         }
       }
+      stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+      stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
     }
-    stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
-    stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
   }
   stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
   stage_flush(fstage, fst_n, far_out, far_out_cnt, c.cap, c.ctr);
@@ -1083,7 +1118,7 @@ __global__ void k_far_flip(DevCtx c) {
 void launch_far_refill(const DevCtx& c, int nslots, uint32_t n_far, uint64_t* near,
                        uint32_t* near_cnt, hipStream_t st, bool selective) {
   if (nslots <= 0 || n_far == 0) return;
-  const uint32_t blocks = (n_far + BLOCK - 1) / BLOCK;
+  const uint32_t blocks = (n_far + BLOCK * FPL - 1) / (BLOCK * FPL);
   const int grid = (int)(blocks < 8192 ? blocks : 8192);
   hipLaunchKernelGGL(k_far_min, grid, BLOCK, 0, st, c);
   hipLaunchKernelGGL(k_far_thr, 1, 256, 0, st, c, selective ? 1 : 0);
@@ -1125,20 +1160,28 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* nea
   const int sel = *c.far_sel;
   const uint32_t n = min(c.far_cnt[sel], c.cap);
   const uint64_t* far = c.far[sel];
-  for (uint32_t base = blockIdx.x * BLOCK; base < n; base += gridDim.x * BLOCK) {
-    const uint32_t i = base + threadIdx.x;
-    uint32_t want = 0;
-    uint64_t item = 0;
-    if (i < n) {
-      const uint64_t it = far[i];
-      const Sched& x = c.sched[it_s(it)];
-      if (x.prev < x.thr) {  // the slot's band grew: v continues
-        want = it_ch(it) + 1u;
-        // continuation flag in the subset field, chunk field cleared for the chunk index
-        item = (it & ~((uint64_t)0xFFFFu << 36)) | ((uint64_t)0x80u << 24);
-      }
+  for (uint32_t base = blockIdx.x * BLOCK * FPL; base < n; base += gridDim.x * BLOCK * FPL) {
+    uint64_t itq[FPL];
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      const uint32_t i = base + q * BLOCK + threadIdx.x;
+      itq[q] = i < n ? far[i] : 0;
     }
-    stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      uint32_t want = 0;
+      uint64_t item = 0;
+      if (base + q * BLOCK + threadIdx.x < n) {
+        const uint64_t it = itq[q];
+        const Sched& x = c.sched[it_s(it)];
+        if (x.prev < x.thr) {  // the slot's band grew: v continues
+          want = it_ch(it) + 1u;
+          // continuation flag in the subset field, chunk field cleared for the chunk index
+          item = (it & ~((uint64_t)0xFFFFu << 36)) | ((uint64_t)0x80u << 24);
+        }
+      }
+      stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+    }
   }
   stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
 }
@@ -1146,7 +1189,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* nea
 void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
                             hipStream_t st) {
   if (n_far == 0) return;
-  const uint32_t blocks = (n_far + BLOCK - 1) / BLOCK;
+  const uint32_t blocks = (n_far + BLOCK * FPL - 1) / (BLOCK * FPL);
   const int grid = (int)(blocks < 8192 ? blocks : 8192);
   hipLaunchKernelGGL(k_sssp_band_grow, 1, 256, 0, st, c);
   hipLaunchKernelGGL(k_sssp_far_move, grid, BLOCK, 0, st, c, near, near_cnt);
