@@ -16,7 +16,10 @@ namespace gst {
 constexpr int32_t INF = 1 << 29;
 constexpr int BLOCK = 256;          // threads per workgroup of the expand kernels
 constexpr int WAVE = 64;            // gfx950 wavefront
-constexpr int RSTRIDE = 72;         // ints per subset in the route tables: 8x8 route + 8 min-route
+// ints per subset X in the route tables: 8x8 routes, 8 row minima, and at
+// RLBMIN the smallest completion bound lb(v, X) over all vertices v
+constexpr int RSTRIDE = 73;
+constexpr int RLBMIN = 72;
 constexpr int RTABLE = 128 * RSTRIDE;  // route ints per query slot (2^7 subsets)
 constexpr int CSTAMP = 7;              // word of a distance row holding the one-label stamp
 constexpr int CROW = 8;             // one-label distances per vertex (k <= 7, padded to 8)

@@ -1271,6 +1271,7 @@ __global__ __launch_bounds__(64) void k_route(DevCtx c) {
       for (int jj = 0; jj < k; ++jj) m = min(m, rt[(tid * 8 + jj) * 128 + X]);
       R[X * RSTRIDE + 64 + tid] = m;
     }
+    if (tid == 0) R[X * RSTRIDE + RLBMIN] = INF;  // lowered by k_lb_table
   }
 }
 
@@ -1315,6 +1316,8 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
   const int s = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   const SlotDesc S = c.slots[s];
   __shared__ int32_t s_r[128 * RC];
+  __shared__ int32_t s_lbmin[128];  // the workgroup's smallest bound per U
+  if (threadIdx.x < 128) s_lbmin[threadIdx.x] = INF;
   for (int x = threadIdx.x; x < S.nsub * RC; x += BLOCK) {
     const int X = x / RC, e = x - X * RC;
     const int src = e < 49 ? (e / 7) * 8 + e % 7 : 64 + (e - 49);
@@ -1381,12 +1384,17 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
             lb2[r] = max(lb2[r], (cs[r][i] + rmin + mn[r]) >> 1);
           }
         }
+        int32_t wmin = INF;
 #pragma unroll
         for (int r = 0; r < VPL; ++r) {
           const int32_t lb =
               U == 0 ? 0 : (one[r] >= INF ? one[r] : max(one[r], max(lb1[r], lb2[r])));
           tile[(lane + WAVE * r) * TS + (p - h)] = lb;
+          if (v0 + lane + WAVE * r < c.V) wmin = min(wmin, lb);
         }
+#pragma unroll
+        for (int off = WAVE / 2; off > 0; off >>= 1) wmin = min(wmin, __shfl_down(wmin, off, WAVE));
+        if (lane == 0 && wmin < s_lbmin[U]) atomicMin(&s_lbmin[U], wmin);
       }
       __syncthreads();
       // one lane per (vertex, two consecutive subsets): a 16-byte store of two
@@ -1409,6 +1417,11 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
       __syncthreads();
     }
   }
+  // the slot's smallest bound per U (k_route set it to INF): a grown state of
+  // subset p needs cost + lb <= best + 2, so the DP cuts its edges at
+  // best + 2 - t - min_v lb(v, full ^ p)
+  if (threadIdx.x < S.nsub && s_lbmin[threadIdx.x] < INF)
+    atomicMin(&c.R[S.r_off + (int64_t)threadIdx.x * RSTRIDE + RLBMIN], s_lbmin[threadIdx.x]);
 }
 
 void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st, const int32_t* sel) {
@@ -1587,13 +1600,15 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
       if (expand) {
         if (!sc->active) sc->active = 1;
         nF += ch == 0;
-        // a grown state costlier than best + 2 (hop: best) can neither be
-        // queued nor lie on a solution better than best: skip those edges
+        // a grown state whose cost plus completion bound exceeds best + 2
+        // (hop: best) is never written: its bound lb(u, full ^ p) is at least
+        // the slot's smallest for that subset, so edges heavier than
+        // best + 2 - t - that minimum are skipped
         const int32_t bnow = best_of(&c.bestw[s]);
         int32_t limit = INF;
-        if (bnow < INF) {
-          if (!HOP) limit = bnow + 2 - t;
-          else if (c.hop_prune) limit = bnow - t;
+        if (bnow < INF && (!HOP || c.hop_prune)) {
+          const int32_t lbmin = c.R[S.r_off + (int64_t)((uint32_t)S.full ^ p) * RSTRIDE + RLBMIN];
+          limit = (HOP ? bnow : bnow + 2) - t - min(lbmin, INF);
         }
         const int32_t rb = c.rowp[v] + (int32_t)(ch << c.chunk_shift);
         const int32_t re = row_cut(c.wt, rb, min(c.rowp[v + 1], rb + CH), limit);
