@@ -144,7 +144,7 @@ struct gst_graph {
   int32_t sssp_band_eff = 0;   // the value in force for the batch being solved
   int32_t dp_div = -1;     // -1 = automatic (kAutoBandDiv for batches of kAutoBandSlots+)
   int32_t dp_div_eff = 0;  // the value in force for the batch being solved
-  int32_t radius_pct = 100;
+  int32_t radius_pct = 0;  // 0: 85 for queries of up to 6 groups, 100 for 7
   int32_t max_batch = 0;
   int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
   int32_t list_permille = 0;  // worklist capacity override (0 = default sizing)
@@ -1813,7 +1813,7 @@ int gst_set_option(gst_graph* g, int32_t opt, int64_t v) {
       g->max_batch = (int32_t)v;
       break;
     case GST_OPT_RADIUS_PCT:
-      if (v < 1 || v > 100000) return fail(GST_ERR_INVALID, "radius percentage out of range");
+      if (v < 0 || v > 100000) return fail(GST_ERR_INVALID, "radius percentage out of range");
       g->radius_pct = (int32_t)v;
       break;
     case GST_OPT_TRACE:

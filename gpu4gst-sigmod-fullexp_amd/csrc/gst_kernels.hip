@@ -168,10 +168,13 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
 // One-label radius of a plain query: distances at or beyond it are never
 // propagated and are clamped to it afterwards (a valid lower bound). Any
 // radius keeps every bound valid; radius_pct only trades one-label work
-// against lower-bound strength.
-__device__ inline int32_t sssp_radius(const DevCtx& c, int32_t best) {
+// against lower-bound strength. 0 (automatic): 85% of the upper bound for
+// queries of up to six groups, 100% for seven, whose DP over 128 subsets
+// depends most on the bounds.
+__device__ inline int32_t sssp_radius(const DevCtx& c, int32_t k, int32_t best) {
   if (best >= INF - 2) return INF;
-  return (int32_t)(((int64_t)best * c.radius_pct) / 100) + 2;
+  const int32_t pct = c.radius_pct > 0 ? c.radius_pct : (k >= 7 ? 100 : 85);
+  return (int32_t)(((int64_t)best * pct) / 100) + 2;
 }
 
 // Rows are sorted by weight: first position in [lo, hi) whose weight exceeds
@@ -563,7 +566,7 @@ __global__ __launch_bounds__(BLOCK) void k_sssp_expand(DevCtx c, WL w) {
       const SlotDesc S = c.slots[s];
       if (c.trace && !c.sched[s].active) c.sched[s].active = 1;  // per-iteration trace only
       const int32_t best = S.D < 0 ? best_of(&c.bestw[s]) : INF;
-      const int32_t radius = sssp_radius(c, best);
+      const int32_t radius = sssp_radius(c, S.k, best);
       n_noub += S.D < 0 && radius >= INF;
       const int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
       int4 a = *reinterpret_cast<const int4*>(row);
@@ -883,7 +886,7 @@ __global__ void k_clamp(DevCtx c) {
   int32_t R = INF;
   if (S.D < 0) {
     const int32_t best = best_of(&c.bestw[s]);
-    R = sssp_radius(c, best);
+    R = sssp_radius(c, S.k, best);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) c.sched[s].R = R;
   if (R >= INF) return;
@@ -1144,7 +1147,7 @@ __global__ void k_sssp_band_grow(DevCtx c) {
   const int ns = *c.nslots;
   for (int s = threadIdx.x; s < ns; s += blockDim.x) {
     Sched& x = c.sched[s];
-    const int32_t radius = sssp_radius(c, best_of(&c.bestw[s]));
+    const int32_t radius = sssp_radius(c, c.slots[s].k, best_of(&c.bestw[s]));
     x.prev = x.thr;
     if (x.thr < radius) x.thr = min(x.thr < INF / 2 ? 2 * x.thr : INF, radius);
   }

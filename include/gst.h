@@ -118,8 +118,9 @@ typedef struct gst_stats {
 #define GST_OPT_SSSP_DELTA 10      /* one-label phase in bands of label value: first band = value,
                                       doubling up to the radius (hop-bounded: until every label
                                       is final); 0 = unordered; -1 (default) = 16 */
-#define GST_OPT_RADIUS_PCT 12      /* one-label radius = upper bound * value / 100 + 2 (default 100);
-                                      distances beyond it are clamped lower bounds */
+#define GST_OPT_RADIUS_PCT 12      /* one-label radius = upper bound * value / 100 + 2; distances
+                                      beyond it are clamped lower bounds. 0 (default) =
+                                      automatic: 85 for queries of up to 6 groups, 100 for 7 */
 #define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */
 #define GST_OPT_DP_DELTA_DIV 11    /* plain DP in A* bands of cost + completion bound: first band
                                       = upper bound / value, doubling; 0 = unordered; -1 (default)

@@ -231,6 +231,29 @@ def test_ablation_switches(gpu, lane, per_item):
             _check_tree(g, grp, q, res.costs[i], res.tree(i), Dq)
 
 
+@pytest.mark.parametrize("pct", [0, 40, 85, 100, 150])
+def test_one_label_radius_never_changes_a_cost(gpu, pct):
+    # the radius only trades one-label work against bound strength (0 is the
+    # automatic rule: 85% for up to six groups, 100% for seven); the grow cut
+    # by each subset's smallest bound is exercised at every radius
+    g, grp = workloads.small(220, 700, 8, seed_=73, whi=90, member_p=0.04)
+    G = _graph(g, grp)
+    G.set_option("radius_pct", pct)
+    rng = np.random.default_rng(79)
+    qs = [rng.choice(8, int(k), replace=False).astype(np.int32) for k in rng.integers(2, 8, 40)]
+    _say(f"one-label radius {pct}%: 40 plain queries n={g.n}")
+    b = G.solve_batch(qs)
+    for i, q in enumerate(qs):
+        ref = O.dpbf(g, grp, q)
+        if ref["status"] == O.INFEASIBLE:
+            assert b.status[i] == STATUS_INFEASIBLE
+            continue
+        assert b.costs[i] == ref["cost"], (pct, i, b.costs[i], ref["cost"])
+        _check_tree(g, grp, q, b.costs[i], b.tree(i), -1)
+    r = G.solve(qs[0])
+    assert r.cost == b.costs[0]
+
+
 def test_list_overflow_reruns_exactly(gpu):
     # lists far too small for the batch: some list (near, far, one-label or DP)
     # overflows, the batch is re-run with full-size lists, results stay exact;
