@@ -1290,18 +1290,22 @@ void launch_route(const DevCtx& c, int nslots, hipStream_t st) {
 // with exact group-route tables). A relaxation then decides from one 8-byte
 // read whether its target can still beat the upper bound.
 #ifndef GST_LB_TP
-#define GST_LB_TP 16
+#define GST_LB_TP 32
 #endif
+#ifndef GST_LB_BLOCK
+#define GST_LB_BLOCK 512
+#endif
+constexpr int LBB = GST_LB_BLOCK;  // threads per workgroup of the bound kernel
 typedef long long i64x2 __attribute__((vector_size(16)));  // two state words
 #ifndef GST_LB_VPL
 #define GST_LB_VPL 2
 #endif
-__global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) {
-  // A workgroup stages its slot's route tables in LDS (entries for groups
-  // 0..6 only: 49 pair routes + 7 row minima per subset) and then walks tiles
-  // of 128 consecutive vertices, subsets in passes of TP (a smaller tile than
-  // all 128 subsets at once lets four workgroups share a CU). Each wave
-  // takes every fourth subset p, so U = full ^ p is the same in all its lanes
+__global__ __launch_bounds__(LBB) void k_lb_table(DevCtx c, const int32_t* sel) {
+  // A workgroup (eight waves) stages its slot's route tables in LDS (entries
+  // for groups 0..6 only: 49 pair routes + 7 row minima per subset) and then
+  // walks tiles of 128 consecutive vertices, subsets in passes of TP = 32 (a
+  // smaller tile than all 128 subsets at once lets three workgroups share a
+  // CU). Each wave takes every eighth subset p, so U = full ^ p is the same in all its lanes
   // (two vertices per lane, which each route read serves) and every route
   // read is an LDS broadcast. The loop
   // over the first group i of a pair branches (wave-uniformly) on i in U; the
@@ -1321,7 +1325,7 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
   __shared__ int32_t s_r[128 * RC];
   __shared__ int32_t s_lbmin[128];  // the workgroup's smallest bound per U
   if (threadIdx.x < 128) s_lbmin[threadIdx.x] = INF;
-  for (int x = threadIdx.x; x < S.nsub * RC; x += BLOCK) {
+  for (int x = threadIdx.x; x < S.nsub * RC; x += LBB) {
     const int X = x / RC, e = x - X * RC;
     const int src = e < 49 ? (e / 7) * 8 + e % 7 : 64 + (e - 49);
     s_r[x] = e < 56 ? c.R[S.r_off + X * RSTRIDE + src] : INF;
@@ -1346,7 +1350,7 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
     const int nv = (int)min<int64_t>(TV, c.V - v0);
     for (int h = 0; h < S.nsub; h += TP) {
       const int np = min(TP, S.nsub - h);  // a power of two
-      for (int p = h + wv; p < h + np; p += BLOCK / WAVE) {
+      for (int p = h + wv; p < h + np; p += LBB / WAVE) {
         if (p == 0) {  // p = 0 is no state: its word is written as {INF, 0}
           for (int r = 0; r < VPL; ++r) tile[(lane + WAVE * r) * TS] = 0;
           continue;
@@ -1407,7 +1411,7 @@ __global__ __launch_bounds__(256) void k_lb_table(DevCtx c, const int32_t* sel) 
       const int sh = __builtin_ctz((unsigned)np) - 1;
       const int64_t tb = t_ix(S, h, v0);
 #pragma unroll 4
-      for (int x = threadIdx.x; x < (nv << sh); x += BLOCK) {
+      for (int x = threadIdx.x; x < (nv << sh); x += LBB) {
         const int vl = x >> sh, pp = 2 * (x & ((np >> 1) - 1));
         const uint32_t lb0 = (uint32_t)tile[vl * TS + pp], lb1 = (uint32_t)tile[vl * TS + pp + 1];
         const i64x2 pair = {(long long)(((uint64_t)lb0 << 32) | (uint32_t)INF),
@@ -1434,7 +1438,7 @@ void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st, const int32_t*
   int per_slot = (8192 + nslots - 1) / nslots;
   if (per_slot < 4) per_slot = 4;
   dim3 grid(tiles < per_slot ? tiles : per_slot, nslots);
-  hipLaunchKernelGGL(k_lb_table, grid, BLOCK, 0, st, c, sel);
+  hipLaunchKernelGGL(k_lb_table, grid, LBB, 0, st, c, sel);
 }
 
 // ---------------------------------------------------------------------------
