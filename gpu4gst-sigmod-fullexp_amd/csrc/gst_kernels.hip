@@ -1323,7 +1323,7 @@ __global__ __launch_bounds__(LBB) void k_lb_table(DevCtx c, const int32_t* sel) 
   const int s = sel ? sel[blockIdx.y] : (int)blockIdx.y;
   const SlotDesc S = c.slots[s];
   __shared__ int32_t s_r[128 * RC];
-  __shared__ int32_t s_lbmin[128];  // the workgroup's smallest bound per U
+  __shared__ int32_t s_lbmin[128];  // the workgroup's smallest bound per subset p
   if (threadIdx.x < 128) s_lbmin[threadIdx.x] = INF;
   for (int x = threadIdx.x; x < S.nsub * RC; x += LBB) {
     const int X = x / RC, e = x - X * RC;
@@ -1391,17 +1391,12 @@ __global__ __launch_bounds__(LBB) void k_lb_table(DevCtx c, const int32_t* sel) 
             lb2[r] = max(lb2[r], (cs[r][i] + rmin + mn[r]) >> 1);
           }
         }
-        int32_t wmin = INF;
 #pragma unroll
         for (int r = 0; r < VPL; ++r) {
           const int32_t lb =
               U == 0 ? 0 : (one[r] >= INF ? one[r] : max(one[r], max(lb1[r], lb2[r])));
           tile[(lane + WAVE * r) * TS + (p - h)] = lb;
-          if (v0 + lane + WAVE * r < c.V) wmin = min(wmin, lb);
         }
-#pragma unroll
-        for (int off = WAVE / 2; off > 0; off >>= 1) wmin = min(wmin, __shfl_down(wmin, off, WAVE));
-        if (lane == 0 && wmin < s_lbmin[U]) atomicMin(&s_lbmin[U], wmin);
       }
       __syncthreads();
       // one lane per (vertex, two consecutive subsets): a 16-byte store of two
@@ -1410,17 +1405,25 @@ __global__ __launch_bounds__(LBB) void k_lb_table(DevCtx c, const int32_t* sel) 
       // lanes consecutive pairs
       const int sh = __builtin_ctz((unsigned)np) - 1;
       const int64_t tb = t_ix(S, h, v0);
+      // the lane's pair of subsets is the same in every round (the workgroup
+      // size is a multiple of np / 2): it keeps their minima over the tile
+      int32_t m0 = INF, m1 = INF;
 #pragma unroll 4
       for (int x = threadIdx.x; x < (nv << sh); x += LBB) {
         const int vl = x >> sh, pp = 2 * (x & ((np >> 1) - 1));
-        const uint32_t lb0 = (uint32_t)tile[vl * TS + pp], lb1 = (uint32_t)tile[vl * TS + pp + 1];
-        const i64x2 pair = {(long long)(((uint64_t)lb0 << 32) | (uint32_t)INF),
-                            (long long)(((uint64_t)lb1 << 32) | (uint32_t)INF)};
+        const int32_t lb0 = tile[vl * TS + pp], lb1 = tile[vl * TS + pp + 1];
+        m0 = min(m0, lb0);
+        m1 = min(m1, lb1);
+        const i64x2 pair = {(long long)(((uint64_t)(uint32_t)lb0 << 32) | (uint32_t)INF),
+                            (long long)(((uint64_t)(uint32_t)lb1 << 32) | (uint32_t)INF)};
         int64_t ix = tb + (int64_t)vl * nrows + pp;
         // streaming stores: the table is far larger than the caches
         for (int dd = 0; dd < nd; ++dd, ix += S.nsub)
           __builtin_nontemporal_store(pair, reinterpret_cast<i64x2*>(&T2[ix]));
       }
+      const int pl = h + 2 * (threadIdx.x & ((np >> 1) - 1));
+      if (m0 < s_lbmin[pl]) atomicMin(&s_lbmin[pl], m0);
+      if (m1 < s_lbmin[pl + 1]) atomicMin(&s_lbmin[pl + 1], m1);
       __syncthreads();
     }
   }
@@ -1428,7 +1431,8 @@ __global__ __launch_bounds__(LBB) void k_lb_table(DevCtx c, const int32_t* sel) 
   // subset p needs cost + lb <= best + 2, so the DP cuts its edges at
   // best + 2 - t - min_v lb(v, full ^ p)
   if (threadIdx.x < S.nsub && s_lbmin[threadIdx.x] < INF)
-    atomicMin(&c.R[S.r_off + (int64_t)threadIdx.x * RSTRIDE + RLBMIN], s_lbmin[threadIdx.x]);
+    atomicMin(&c.R[S.r_off + (int64_t)((uint32_t)S.full ^ threadIdx.x) * RSTRIDE + RLBMIN],
+              s_lbmin[threadIdx.x]);
 }
 
 void launch_lb_table(const DevCtx& c, int nslots, hipStream_t st, const int32_t* sel) {
