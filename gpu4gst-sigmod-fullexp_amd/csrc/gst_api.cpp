@@ -314,11 +314,15 @@ int unfold_grid(const gst_graph* g) { return std::min(expand_grid(g), g->cus * 8
 // 256-edge chunks (260 queries/s single vs 190 at 4096); a batch of many
 // queries fills the device with items and coarse chunks keep the per-item
 // overhead low (4096: 2,190 queries/s vs 1,960 at 256); the DP and one-label
-// band rules stay with solve_lockstep's defaults.
+// band rules stay with solve_lockstep's defaults. A batch of 32 or more
+// queries gets four workgroups per CU, the number the expand kernels keep
+// resident (four waves per SIMD): with eight, their grid-stride loops run in
+// two rounds and the second round's tail shows (3,790 -> 3,950 queries/s;
+// six per CU: 3,450).
 Tune model_settings(const gst_graph* g, int cls) {
   Tune t;
   t.chunk_shift = cls == 0 ? 8 : (cls == 1 ? 10 : 12);
-  t.grid = g->cus * 8;
+  t.grid = cls == 2 ? g->cus * 4 : g->cus * 8;
   return t;
 }
 
@@ -1946,7 +1950,7 @@ int gst_autotune(gst_graph* g, const int64_t* q_offsets, const int32_t* q_groups
   Tune cur = model_settings(g, cls);
   if (g->tuned_ok[cls]) cur = g->tuned[cls];
   if (cur.chunk_shift < 0) cur.chunk_shift = model_settings(g, cls).chunk_shift;
-  if (cur.grid <= 0) cur.grid = g->cus * 8;
+  if (cur.grid <= 0) cur.grid = model_settings(g, cls).grid;
   gst_tuning res;
   std::memset(&res, 0, sizeof(res));
   res.size_class = cls;

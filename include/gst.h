@@ -240,7 +240,7 @@ int gst_backward(gst_graph* g, int32_t nq, const int64_t* d_edge_offsets,
 /* Autotuner. The expand kernels' settings (virtual-split chunk, workgroup
  * count, first DP band divisor, first one-label band width) are chosen per
  * size class of a call: one query, 2..31 queries, 32 or more. By default a
- * model of the class picks them (chunk 256 / 1024 / 4096 edges, 8
+ * model of the class picks them (chunk 256 / 1024 / 4096 edges, 8 / 8 / 4
  * workgroups per CU, the band rules of GST_OPT_DP_DELTA_DIV and
  * GST_OPT_SSSP_DELTA); options set with gst_set_option always win.
  * gst_autotune measures instead: on the given sample of queries it times
