@@ -1155,38 +1155,54 @@ __global__ void k_sssp_band_grow(DevCtx c) {
   if (threadIdx.x == 0) *c.band_key = *c.iter + 1;  // the next band's first iteration + 1
 }
 
+// The continuations of a round (FPL far items per lane) take one output range
+// per workgroup, reserved with a single atomic on the list counter: the
+// counter is one word that every workgroup hits, and this kernel does nothing
+// but append.
 __global__ __launch_bounds__(BLOCK) void k_sssp_far_move(DevCtx c, uint64_t* near,
                                                          uint32_t* near_cnt) {
-  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
-  uint64_t* stage = s_stage[threadIdx.x / WAVE];
-  uint32_t st_n = 0;
+  __shared__ int32_t s_off[BLOCK + 1];
+  __shared__ int32_t s_wsum[BLOCK / WAVE];
+  __shared__ uint32_t s_base;
+  const int tid = threadIdx.x;
   const int sel = *c.far_sel;
   const uint32_t n = min(c.far_cnt[sel], c.cap);
   const uint64_t* far = c.far[sel];
   for (uint32_t base = blockIdx.x * BLOCK * FPL; base < n; base += gridDim.x * BLOCK * FPL) {
     uint64_t itq[FPL];
+    uint32_t wq[FPL];
+    int32_t tot = 0;
 #pragma unroll
     for (int q = 0; q < FPL; ++q) {
-      const uint32_t i = base + q * BLOCK + threadIdx.x;
+      const uint32_t i = base + (uint32_t)(tid * FPL + q);  // a lane's items are adjacent
       itq[q] = i < n ? far[i] : 0;
     }
 #pragma unroll
     for (int q = 0; q < FPL; ++q) {
-      uint32_t want = 0;
-      uint64_t item = 0;
-      if (base + q * BLOCK + threadIdx.x < n) {
+      wq[q] = 0;
+      if (base + (uint32_t)(tid * FPL + q) < n) {
         const uint64_t it = itq[q];
         const Sched& x = c.sched[it_s(it)];
         if (x.prev < x.thr) {  // the slot's band grew: v continues
-          want = it_ch(it) + 1u;
+          wq[q] = it_ch(it) + 1u;
           // continuation flag in the subset field, chunk field cleared for the chunk index
-          item = (it & ~((uint64_t)0xFFFFu << 36)) | ((uint64_t)0x80u << 24);
+          itq[q] = (it & ~((uint64_t)0xFFFFu << 36)) | ((uint64_t)0x80u << 24);
         }
       }
-      stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
+      tot += (int32_t)wq[q];
     }
+    block_scan(tot, s_off, s_wsum);
+    if (tid == 0) s_base = s_off[BLOCK] ? atomicAdd(near_cnt, (uint32_t)s_off[BLOCK]) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + (uint32_t)s_off[tid];
+#pragma unroll
+    for (int q = 0; q < FPL; ++q)
+      for (uint32_t r = 0; r < wq[q]; ++r, ++pos) {
+        if (pos < c.cap) near[pos] = itq[q] | ((uint64_t)r << 36);
+        else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+      }
+    __syncthreads();  // s_off and s_base are reused by the next round
   }
-  stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
 }
 
 void launch_sssp_far_refill(const DevCtx& c, uint32_t n_far, uint64_t* near, uint32_t* near_cnt,
