@@ -1053,12 +1053,14 @@ __global__ void k_far_thr(DevCtx c, int selective) {
   }
 }
 
+// Both outputs of a round take one range per workgroup (one scan of the
+// packed counts, near in the low and far in the high 16 bits; at most
+// BLOCK * FPL each), reserved with one atomic per list.
 __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, uint32_t* near_cnt) {
-  __shared__ uint64_t s_stage[BLOCK / WAVE][WSTAGE];
-  __shared__ uint64_t s_fstage[BLOCK / WAVE][WSTAGE];
-  uint64_t* stage = s_stage[threadIdx.x / WAVE];
-  uint64_t* fstage = s_fstage[threadIdx.x / WAVE];
-  uint32_t st_n = 0, fst_n = 0;
+  __shared__ int32_t s_off[BLOCK + 1];
+  __shared__ int32_t s_wsum[BLOCK / WAVE];
+  __shared__ uint32_t s_base[2];
+  const int tid = threadIdx.x;
   const int sel = *c.far_sel;
   const uint32_t n = min(c.far_cnt[sel], c.cap);
   const uint64_t* far = c.far[sel];
@@ -1069,47 +1071,61 @@ __global__ __launch_bounds__(BLOCK) void k_far_move(DevCtx c, uint64_t* near, ui
     int2 tlq[FPL];
 #pragma unroll
     for (int q = 0; q < FPL; ++q) {
-      const uint32_t i = base + q * BLOCK + threadIdx.x;
+      const uint32_t i = base + q * BLOCK + tid;
       itq[q] = i < n ? far[i] : 0;
     }
 #pragma unroll
     for (int q = 0; q < FPL; ++q)
-      if (base + q * BLOCK + threadIdx.x < n) tlq[q] = far_word(c, itq[q]);
+      if (base + q * BLOCK + tid < n) tlq[q] = far_word(c, itq[q]);
+    uint32_t wmask = 0, fmask = 0;  // items of this lane for the near / far list
 #pragma unroll
     for (int q = 0; q < FPL; ++q) {
-      uint32_t want = 0, fwant = 0;
-      uint64_t item = 0, fitem = 0;
-      if (base + q * BLOCK + threadIdx.x < n) {
-        const uint64_t it = itq[q];
-        int32_t f;
-        bool half_drop = false;
-        const bool live = far_live(c, it, tlq[q], &f, &half_drop);
-        if (half_drop) {  // never expanded now: combine it with its complement first
-          const SlotDesc S = c.slots[it_s(it)];
-          complement_merge(c, S, it_s(it), it_v(it), it_p(it),
-                           c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
-        }
-        if (live) {
-          const uint32_t v = it_v(it), s = it_s(it);
-          const SlotDesc S = c.slots[s];
-          if (f < c.sched[s].thr) {
-            // back to the frontier; duplicates of a state in the far list collapse here
-            if (mark_state(c, S, v, it_p(it), it_d(it))) {
-              want = 1;
-              item = mk_item(v, 0, it_d(it), 0, s);
-            }
-          } else {
-            fwant = 1;
-            fitem = it;
-          }
-        }
+      if (base + q * BLOCK + tid >= n) continue;
+      const uint64_t it = itq[q];
+      int32_t f;
+      bool half_drop = false;
+      const bool live = far_live(c, it, tlq[q], &f, &half_drop);
+      if (half_drop) {  // never expanded now: combine it with its complement first
+        const SlotDesc S = c.slots[it_s(it)];
+        complement_merge(c, S, it_s(it), it_v(it), it_p(it),
+                         c.T[2 * t_ix(S, item_row(S, it), it_v(it))]);
       }
-      stage_append(stage, st_n, want, item, near, near_cnt, c.cap, c.ctr);
-      stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr);
+      if (!live) continue;
+      const uint32_t v = it_v(it), s = it_s(it);
+      const SlotDesc S = c.slots[s];
+      if (f < c.sched[s].thr) {
+        // back to the frontier; duplicates of a state in the far list collapse here
+        if (mark_state(c, S, v, it_p(it), it_d(it))) {
+          wmask |= 1u << q;
+          itq[q] = mk_item(v, 0, it_d(it), 0, s);
+        }
+      } else {
+        fmask |= 1u << q;
+      }
     }
+    block_scan(__popc(wmask) | (__popc(fmask) << 16), s_off, s_wsum);
+    if (tid == 0) {
+      const uint32_t tot = (uint32_t)s_off[BLOCK], tn = tot & 0xFFFFu, tf = tot >> 16;
+      s_base[0] = tn ? atomicAdd(near_cnt, tn) : 0u;
+      s_base[1] = tf ? atomicAdd(far_out_cnt, tf) : 0u;
+    }
+    __syncthreads();
+    uint32_t pn = s_base[0] + ((uint32_t)s_off[tid] & 0xFFFFu);
+    uint32_t pf = s_base[1] + ((uint32_t)s_off[tid] >> 16);
+#pragma unroll
+    for (int q = 0; q < FPL; ++q) {
+      if ((wmask >> q) & 1u) {
+        if (pn < c.cap) near[pn] = itq[q];
+        else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+        ++pn;
+      } else if ((fmask >> q) & 1u) {
+        if (pf < c.cap) far_out[pf] = itq[q];
+        else atomicAdd(&c.ctr[C_OVERFLOW], 1ull);
+        ++pf;
+      }
+    }
+    __syncthreads();  // s_off and s_base are reused by the next round
   }
-  stage_flush(stage, st_n, near, near_cnt, c.cap, c.ctr);
-  stage_flush(fstage, fst_n, far_out, far_out_cnt, c.cap, c.ctr);
 }
 
 __global__ void k_far_flip(DevCtx c) {
