@@ -305,8 +305,11 @@ int expand_grid(const gst_graph* g) { return g->grid_eff > 0 ? g->grid_eff : g->
 // The unfold step: one LDS search per state item it writes; the hop-bounded
 // DP's vertex items carry many pending states, and eight workgroups per CU
 // hide those latencies (hop DP phase 1,655 -> 1,589 ms on the 1.1M-vertex
-// workload; no change on the plain 168k-vertex one).
-int unfold_grid(const gst_graph* g) { return std::min(expand_grid(g), g->cus * 8); }
+// workload; no change on the plain 168k-vertex one), whatever the expand
+// kernels use, unless GST_OPT_GRID sets fewer.
+int unfold_grid(const gst_graph* g) {
+  return g->grid > 0 ? std::min(g->grid, g->cus * 8) : g->cus * 8;
+}
 
 // Size-class model of the expand settings, from sweeps on the 168k-vertex /
 // 6.8M-edge power-law workload (chunk 128..8192, grid 2..16 x CUs): a lone
