@@ -144,7 +144,7 @@ struct gst_graph {
   int32_t sssp_band_eff = 0;   // the value in force for the batch being solved
   int32_t dp_div = -1;     // -1 = automatic (kAutoBandDiv for batches of kAutoBandSlots+)
   int32_t dp_div_eff = 0;  // the value in force for the batch being solved
-  int32_t radius_pct = 0;  // 0: 85 for queries of up to 6 groups, 100 for 7
+  int32_t radius_pct = 0;  // 0: 85 for queries of up to 6 groups, 95 for 7
   int32_t max_batch = 0;
   int32_t sssp_slots = 0;  // slots per one-label pass (0 = the whole batch)
   int32_t list_permille = 0;  // worklist capacity override (0 = default sizing)

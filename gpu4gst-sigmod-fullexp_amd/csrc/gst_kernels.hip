@@ -169,11 +169,11 @@ __device__ inline uint32_t nchunks(const DevCtx& c, uint32_t u) {
 // propagated and are clamped to it afterwards (a valid lower bound). Any
 // radius keeps every bound valid; radius_pct only trades one-label work
 // against lower-bound strength. 0 (automatic): 85% of the upper bound for
-// queries of up to six groups, 100% for seven, whose DP over 128 subsets
+// queries of up to six groups, 95% for seven, whose DP over 128 subsets
 // depends most on the bounds.
 __device__ inline int32_t sssp_radius(const DevCtx& c, int32_t k, int32_t best) {
   if (best >= INF - 2) return INF;
-  const int32_t pct = c.radius_pct > 0 ? c.radius_pct : (k >= 7 ? 100 : 85);
+  const int32_t pct = c.radius_pct > 0 ? c.radius_pct : (k >= 7 ? 95 : 85);
   return (int32_t)(((int64_t)best * pct) / 100) + 2;
 }
 

@@ -120,7 +120,7 @@ typedef struct gst_stats {
                                       is final); 0 = unordered; -1 (default) = 16 */
 #define GST_OPT_RADIUS_PCT 12      /* one-label radius = upper bound * value / 100 + 2; distances
                                       beyond it are clamped lower bounds. 0 (default) =
-                                      automatic: 85 for queries of up to 6 groups, 100 for 7 */
+                                      automatic: 85 for queries of up to 6 groups, 95 for 7 */
 #define GST_OPT_MAX_BATCH 14       /* cap on queries per lockstep batch (0 = HBM budget only) */
 #define GST_OPT_DP_DELTA_DIV 11    /* plain DP in A* bands of cost + completion bound: first band
                                       = upper bound / value, doubling; 0 = unordered; -1 (default)

@@ -234,7 +234,7 @@ def test_ablation_switches(gpu, lane, per_item):
 @pytest.mark.parametrize("pct", [0, 40, 85, 100, 150])
 def test_one_label_radius_never_changes_a_cost(gpu, pct):
     # the radius only trades one-label work against bound strength (0 is the
-    # automatic rule: 85% for up to six groups, 100% for seven); the grow cut
+    # automatic rule: 85% for up to six groups, 95% for seven); the grow cut
     # by each subset's smallest bound is exercised at every radius
     g, grp = workloads.small(220, 700, 8, seed_=73, whi=90, member_p=0.04)
     G = _graph(g, grp)
