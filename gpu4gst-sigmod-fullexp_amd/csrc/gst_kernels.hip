@@ -890,16 +890,35 @@ __global__ void k_clamp(DevCtx c) {
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) c.sched[s].R = R;
   if (R >= INF) return;
+  // the row's labels as two 16-byte words; only a changed half is written
   for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < c.V; v += gridDim.x * blockDim.x) {
-    int32_t* row = &c.C[S.c_off + (int64_t)v * CROW];
-    for (int i = 0; i < S.k; ++i)
-      if (row[i] > R) row[i] = R;
+    int4* row = reinterpret_cast<int4*>(&c.C[S.c_off + (int64_t)v * CROW]);
+    int4 a = row[0];
+    if (a.x > R || (S.k > 1 && a.y > R) || (S.k > 2 && a.z > R) || (S.k > 3 && a.w > R)) {
+      a.x = min(a.x, R);
+      if (S.k > 1) a.y = min(a.y, R);
+      if (S.k > 2) a.z = min(a.z, R);
+      if (S.k > 3) a.w = min(a.w, R);
+      row[0] = a;
+    }
+    if (S.k > 4) {
+      int4 b = row[1];  // b.w is the stamp word
+      if (b.x > R || (S.k > 5 && b.y > R) || (S.k > 6 && b.z > R)) {
+        b.x = min(b.x, R);
+        if (S.k > 5) b.y = min(b.y, R);
+        if (S.k > 6) b.z = min(b.z, R);
+        row[1] = b;
+      }
+    }
   }
 }
 
 void launch_clamp(const DevCtx& c, int nslots, hipStream_t st) {
   if (nslots <= 0) return;
-  dim3 grid((c.V + 255) / 256 < 1024 ? (c.V + 255) / 256 : 1024, nslots);
+  // a few thousand workgroups in all, each walking its slot's rows
+  int per_slot = (c.V + 255) / 256 < 8192 / nslots ? (c.V + 255) / 256 : 8192 / nslots;
+  if (per_slot < 1) per_slot = 1;
+  dim3 grid(per_slot, nslots);
   hipLaunchKernelGGL(k_clamp, grid, 256, 0, st, c);
 }
 
