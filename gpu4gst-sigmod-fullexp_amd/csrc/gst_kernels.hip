@@ -39,7 +39,14 @@ __device__ inline int lane_id() { return __lane_id(); }
 #ifndef GST_EPL
 #define GST_EPL 4
 #endif
-constexpr int EPL = GST_EPL;  // edges per lane per round in the expand kernels
+constexpr int EPL = GST_EPL;  // edges per lane per round in the one-label kernel
+#ifndef GST_DEPL
+#define GST_DEPL 2
+#endif
+// ... and in the DP kernel, whose rows are cut short by its bounds: two keep
+// it unspilled at four waves per SIMD (DP 53 -> 51.5 ms on the 168k-vertex
+// workload, hop-bounded DP 1,259 -> 1,225 ms on the 1.1M-vertex one)
+constexpr int DEPL = GST_DEPL;
 #ifndef GST_MPL
 #define GST_MPL 2
 #endif
@@ -1929,16 +1936,16 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
     const int32_t total = s_off[BLOCK];
     const bool per_item = c.per_item_edges != 0;
     const int32_t span = per_item ? block_max(deg, s_wsum) * BLOCK : total;
-    // EPL edges per lane per round, all loads issued before any is used; the
+    // DEPL edges per lane per round, all loads issued before any is used; the
     // next round's CSR reads are issued before this round's state words are
     // used, so each round waits on one memory latency, not two
-    int jq[EPL];
-    uint32_t uq[EPL];
-    int32_t wq[EPL];
-    // item, neighbour and weight of the EPL edges of the round at `b`
+    int jq[DEPL];
+    uint32_t uq[DEPL];
+    int32_t wq[DEPL];
+    // item, neighbour and weight of the DEPL edges of the round at `b`
     auto fetch = [&](int32_t b, int* jo, uint32_t* uo, int32_t* wo) {
 #pragma unroll
-      for (int q = 0; q < EPL; ++q) {
+      for (int q = 0; q < DEPL; ++q) {
         int32_t o;
         if (per_item) {
           o = b / BLOCK + q;
@@ -1958,28 +1965,28 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
       }
     };
     if (span > 0) fetch(0, jq, uq, wq);
-    for (int32_t base = 0; base < span; base += BLOCK * EPL) {
-      int2 dq[EPL];
+    for (int32_t base = 0; base < span; base += BLOCK * DEPL) {
+      int2 dq[DEPL];
 #pragma unroll
-      for (int q = 0; q < EPL; ++q)
+      for (int q = 0; q < DEPL; ++q)
         if (jq[q] >= 0)
           dq[q] = *reinterpret_cast<const int2*>(
               &c.T[2 * (s_dst[jq[q]] + (int64_t)uq[q] * s_rows[jq[q]])]);
-      int jn[EPL];
-      uint32_t un[EPL];
-      int32_t wn[EPL];
-      if (base + BLOCK * EPL < span) {
-        fetch(base + BLOCK * EPL, jn, un, wn);
+      int jn[DEPL];
+      uint32_t un[DEPL];
+      int32_t wn[DEPL];
+      if (base + BLOCK * DEPL < span) {
+        fetch(base + BLOCK * DEPL, jn, un, wn);
       } else {
 #pragma unroll
-        for (int q = 0; q < EPL; ++q) {
+        for (int q = 0; q < DEPL; ++q) {
           jn[q] = -1;
           un[q] = 0;
           wn[q] = 0;
         }
       }
 #pragma unroll
-      for (int q = 0; q < EPL; ++q) {
+      for (int q = 0; q < DEPL; ++q) {
         uint32_t want = 0, fwant = 0;
         uint64_t item = 0, fitem = 0;
         if (jq[q] >= 0) {
@@ -2029,7 +2036,7 @@ __global__ __launch_bounds__(BLOCK, 4) void k_dp_expand(DevCtx c, WL w) {
           stage_append(fstage, fst_n, fwant, fitem, far_out, far_out_cnt, c.cap, c.ctr, lane_ap);
       }
 #pragma unroll
-      for (int q = 0; q < EPL; ++q) {
+      for (int q = 0; q < DEPL; ++q) {
         jq[q] = jn[q];
         uq[q] = un[q];
         wq[q] = wn[q];
