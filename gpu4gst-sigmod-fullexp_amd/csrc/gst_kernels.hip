@@ -37,9 +37,11 @@ namespace {
 __device__ inline int lane_id() { return __lane_id(); }
 
 #ifndef GST_EPL
-#define GST_EPL 4
+#define GST_EPL 3
 #endif
-constexpr int EPL = GST_EPL;  // edges per lane per round in the one-label kernel
+// edges per lane per round in the one-label kernel (3: 111 VGPRs; 4 took 123
+// and 2% longer at four workgroups per CU)
+constexpr int EPL = GST_EPL;
 #ifndef GST_DEPL
 #define GST_DEPL 2
 #endif
